@@ -1,0 +1,127 @@
+"""ctypes binding of libfastipc.so.
+
+The reference reaches its native code the same way:
+``so = ctypes.CDLL(os.path.dirname(__file__) + "/wrapper/a.so")``
+(common/math/external_func.py:1-4).  Here the library is the gfx950 build
+(``python -m fastipc_amd.build``); there is no CPU fallback, so a missing
+library is an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfastipc.so")
+
+_D = C.c_double
+_P = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+_VP = C.c_void_p
+
+STATUS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "capacity exceeded", 4: "non-finite value",
+          5: "PCG not converged", 6: "call out of order"}
+
+
+class FipcError(RuntimeError):
+    def __init__(self, fn, code):
+        super().__init__(f"{fn} failed: {STATUS.get(code, code)} ({code})")
+        self.code = code
+
+
+_SIGS = {
+    "fipc_create": [C.POINTER(_VP)],
+    "fipc_destroy": [_VP],
+    "fipc_upload_mesh": [_VP, C.c_int, _P, _P, C.c_int, _IP, _P, _P, _P, _P, C.c_int, _IP, C.c_int, _IP, C.c_int, _IP,
+                         C.POINTER(C.c_ubyte)],
+    "fipc_set_params": [_VP, _D, _D, _D, _D, _D],
+    "fipc_set_state": [_VP, _P, _P],
+    "fipc_get_state": [_VP, _P, _P],
+    "fipc_begin_step": [_VP, _P],
+    "fipc_end_step": [_VP],
+    "fipc_find_constraints": [_VP, _IP],
+    "fipc_get_constraints": [_VP, C.c_int, _IP],
+    "fipc_min_distance": [_VP, _P],
+    "fipc_assemble": [_VP, C.c_int],
+    "fipc_solve": [_VP, _D, C.c_int, _IP, _P],
+    "fipc_residual": [_VP, _P],
+    "fipc_energy": [_VP, _P, _P],
+    "fipc_ccd_alpha": [_VP, _D, C.c_int, _P, _P],
+    "fipc_save_prev": [_VP],
+    "fipc_apply_sol": [_VP, _D],
+    "fipc_check_collision": [_VP, _IP],
+    "fipc_get_vector": [_VP, C.c_int, _P],
+    "fipc_set_vector": [_VP, C.c_int, _P],
+    "fipc_get_bsr": [_VP, _IP, _IP, _IP, _P],
+    "fipc_spmv": [_VP, _P, _P],
+    "fipc_enable_timing": [_VP, C.c_int],
+    "fipc_kernel_times": [_VP, _P, C.POINTER(C.c_long)],
+    "fipc_sync": [_VP],
+    "fipc_batch_constraint": [C.c_int, C.c_int, _P, _P, _D, _D, C.c_int, _P, _P, _P],
+    "fipc_batch_tet": [C.c_int, _P, _P, _P, _P, _P, _P, _P, _P],
+    "fipc_batch_ccd": [C.c_int, C.c_int, _P, _P, _D, _P, _P],
+}
+_SCALAR = {
+    "project_pd_3": [_D] * 9 + [_P] * 9,
+    "project_pd_6": [_D] * 36 + [_P] * 36,
+    "project_pd_9": [_D] * 81 + [_P] * 81,
+    "project_pd_12": [_D] * 144 + [_P] * 144,
+    "fipc_project_pd": [C.c_int, _P, _P],
+    "svd_3": [_D] * 9 + [_P] * 27,
+    "get_smallest_positive_real_quad_root": [_D] * 4 + [_P],
+    "get_smallest_positive_real_cubic_root": [_D] * 5 + [_P],
+    "point_triangle_ccd": [_D] * 26 + [_P],
+    "edge_edge_ccd": [_D] * 26 + [_P],
+}
+
+#: every symbol include/fastipc.h declares
+EXPORTS = tuple(_SIGS) + tuple(_SCALAR)
+
+_so = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libfastipc.so (once).  Raises if it has not been built."""
+    global _so
+    if _so is not None:
+        return _so
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m fastipc_amd.build`")
+    so = C.CDLL(path)
+    for name, args in _SIGS.items():
+        f = getattr(so, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    for name, args in _SCALAR.items():
+        f = getattr(so, name)
+        f.argtypes = args
+        f.restype = None
+    _so = so
+    return so
+
+
+def call(name, *args):
+    code = getattr(load(), name)(*args)
+    if code != 0:
+        raise FipcError(name, code)
+    return code
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_P)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_IP)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)

@@ -1,0 +1,119 @@
+// fipc_ctx.h -- device-resident state of one simulation (one context per
+// mesh).  Buffers are owned by the context; every entry point is
+// stream-ordered on ctx->s and returns a status code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+#include "fipc_math.h"
+#include "../../include/fastipc.h"
+
+#define NTYPES 7
+
+#define HIPCHK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t _e = (expr);                                                       \
+        if (_e != hipSuccess) {                                                       \
+            fprintf(stderr, "fastipc: HIP error %s at %s:%d\n", hipGetErrorString(_e), \
+                    __FILE__, __LINE__);                                              \
+            return FIPC_ERR_HIP;                                                      \
+        }                                                                             \
+    } while (0)
+
+#define STCHK(expr)                       \
+    do {                                  \
+        int _st = (expr);                 \
+        if (_st != FIPC_OK) return _st;   \
+    } while (0)
+
+template <class T>
+struct DVec {
+    T* p = nullptr;
+    size_t cap = 0;
+};
+
+template <class T>
+static inline int ensure(DVec<T>& v, size_t n)
+{
+    if (n == 0) n = 1;
+    if (v.cap >= n) return FIPC_OK;
+    if (v.p) HIPCHK(hipFree(v.p));
+    v.p = nullptr;
+    size_t c = n + n / 4 + 64;
+    HIPCHK(hipMalloc(&v.p, c * sizeof(T)));
+    v.cap = c;
+    return FIPC_OK;
+}
+template <class T>
+static inline void release(DVec<T>& v)
+{
+    if (v.p) (void)hipFree(v.p);
+    v.p = nullptr;
+    v.cap = 0;
+}
+
+// hashed uniform grid used by every broad phase
+struct Grid {
+    unsigned M = 0;        // bucket count (power of two)
+    DVec<int> cnt;         // per item cell count, scanned in place (n+1)
+    DVec<unsigned> ekey;   // bucket of each entry
+    DVec<int> eitem;       // item of each entry
+    DVec<int> bstart;      // M+1 bucket offsets
+    DVec<int> bfill;       // M cursors
+    DVec<int> items;       // entries ordered by bucket
+    DVec<int> qoff;        // query chunk offsets (nq+1)
+    int nentries = 0;
+};
+
+struct KernelTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    double total_ms = 0.0;
+    long launches = 0;
+};
+
+struct fipc_ctx {
+    hipStream_t s = nullptr;
+    int nv = 0, nt = 0, nbp = 0, nbe = 0, nbt = 0, ndir = 0;
+    // mesh
+    DVec<int> tets, bp, be, bt, dids;
+    DVec<double> IB, vol, la, mu, m;
+    DVec<unsigned char> dmask;
+    // state
+    DVec<double> x, x0, xn, xt, xprev, v, sol, rhs, target;
+    double dt = 0.01, dHat = 1e-3, dHat2 = 1e-6, kappa = 1.0, inv_dx = 1.0, gravity = -9.8;
+    // elastic + inertia BSR (3x3 blocks, values as 9 planes of nnzb)
+    int nnzb = 0;
+    DVec<int> rowptr, col, diag, slots;
+    DVec<double> val;
+    DVec<float> valf;
+    int precision = 0;     // 0 = FP64 Hessian, 1 = FP32 Hessian / SpMV
+    // contacts
+    int64_t cap[NTYPES] = {0};
+    int count[NTYPES] = {0};
+    int raw_count[NTYPES] = {0};
+    DVec<int> rows[NTYPES];
+    DVec<double> cH[NTYPES];
+    DVec<unsigned long long> keys_pp, keys_pe, keys_tmp;
+    DVec<int> counters;    // NTYPES raw append counters + misc
+    DVec<unsigned char> cub_tmp;
+    size_t cub_bytes = 0;
+    DVec<int> nuniq;
+    // preconditioner + PCG
+    DVec<double> cdiag, Minv, r, z, pp, q;
+    DVec<double> part_a, part_b, part_c, scal;
+    DVec<int> pcg_state;   // [0] done_at, [1] iterations
+    // reductions for energy / residual / ccd
+    DVec<double> epart;
+    DVec<unsigned long long> amin;
+    DVec<int> flag;
+    // broad phase grids
+    Grid grid;
+    // statistics
+    double last_pcg_relres = 0.0;
+    int last_pcg_iters = 0;
+    KernelTimer tm_spmv, tm_assemble, tm_ccd, tm_find, tm_energy, tm_contact;
+    bool timing = false;
+};
+
+int fipc_grid_build_points(fipc_ctx* c, int mode, double inflate);

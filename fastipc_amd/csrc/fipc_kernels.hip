@@ -1,0 +1,1265 @@
+// fipc_kernels.hip -- assembly, solve, energy and state kernels of the IPC
+// Newton step for gfx950, and the context entry points that drive them.
+//
+// Data layout in HBM (per context):
+//   x, x0, xn, xTilde, xPrev, v, p, rhs : float64 [nv][3]
+//   elastic + inertia Hessian          : BSR, 3x3 blocks, values as nine
+//                                        planes val[k * nnzb + b] (k = 3r + c)
+//                                        so that lanes walking consecutive
+//                                        blocks load contiguous doubles
+//   contact Hessians                   : per row, the projected (3k x 3k)
+//                                        matrix row-major, k = 2, 3, 4
+//   block-Jacobi preconditioner        : float64 [nv][9] inverse blocks
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstring>
+#include <vector>
+#include "fipc_ctx.h"
+
+using namespace fipc;
+
+#define TPB 256
+
+static inline int nblk(long n, int t = TPB) { return (int)((n + t - 1) / t); }
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double warp_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double warp_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+// deterministic block sum (fixed tree); result valid in thread 0
+__device__ __forceinline__ double block_sum(double v)
+{
+    __shared__ double sh[TPB / 64];
+    v = warp_sum(v);
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) r += sh[i];
+    __syncthreads();
+    return r;
+}
+__device__ __forceinline__ double block_max(double v)
+{
+    __shared__ double sh[TPB / 64];
+    v = warp_max(v);
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) r = fmax(r, sh[i]);
+    __syncthreads();
+    return r;
+}
+// every block reduces the same partial array in the same order
+__device__ __forceinline__ double reduce_partials(const double* part, int n)
+{
+    __shared__ double bc;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) bc = s;
+    __syncthreads();
+    double r = bc;
+    __syncthreads();
+    return r;
+}
+__device__ __forceinline__ v3 ldv(const double* a, int i) { return v3{a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
+
+// ---------------------------------------------------------------------------
+// step state
+// ---------------------------------------------------------------------------
+// compute_xn_and_xTilde (NEWTON_IPC.py:497-502) + move_nodes (:505-511)
+__global__ void k_begin_step(int nv, const double* __restrict__ x, const double* __restrict__ v, double* xn,
+                             double* xt, const unsigned char* dmask, const double* target, double dt, double g)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    for (int d = 0; d < 3; d++) {
+        double xi = x[3 * i + d];
+        xn[3 * i + d] = xi;
+        double t = xi + dt * v[3 * i + d];
+        if (d == 1) t += dt * dt * g;
+        xt[3 * i + d] = dmask[i] ? target[3 * i + d] : t;
+    }
+}
+// compute_v (NEWTON_IPC.py:799-802)
+__global__ void k_end_step(int n3, const double* x, const double* xn, double* v, double inv_dt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) v[i] = (x[i] - xn[i]) * inv_dt;
+}
+// apply_sol (NEWTON_IPC.py:792-796)
+__global__ void k_apply(int n3, const double* xp, const double* p, double* x, double a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) x[i] = xp[i] + p[i] * a;
+}
+// output_residual (NEWTON_IPC.py:805-812): block maxima of |p|
+__global__ void k_maxabs(int n, const double* p, double* part)
+{
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = fmax(m, fabs(p[i]));
+    m = block_max(m);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// ---------------------------------------------------------------------------
+// assembly: inertia + elasticity into the BSR, contacts into per-row blocks
+// ---------------------------------------------------------------------------
+// compute_inertia (NEWTON_IPC.py:558-567): diagonal mass, rhs = -m (x - xTilde)
+__global__ void k_inertia(int nv, int nnzb, const double* __restrict__ m, const double* __restrict__ x,
+                          const double* __restrict__ xt, const int* __restrict__ diag, double* val, double* rhs,
+                          double* cdiag)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    double mi = m[i];
+    int b = diag[i];
+    val[0 * nnzb + b] = mi;
+    val[4 * nnzb + b] = mi;
+    val[8 * nnzb + b] = mi;
+    for (int d = 0; d < 3; d++) rhs[3 * i + d] = -mi * (x[3 * i + d] - xt[3 * i + d]);
+    for (int k = 0; k < 9; k++) cdiag[9 * i + k] = 0.0;
+}
+
+// 12x12 tet Hessian as Z M Z^T (see fipc_math.h NHFactor), accumulated per
+// 3x3 block.  emit(a, c, blk) receives the block of vertex pair (a, c), c >= a.
+template <class Emit>
+__device__ __forceinline__ void tet_hessian_blocks(const NHFactor& f, const double (&Bm)[4][3], double sc, Emit emit)
+{
+    // bv[a][b] = sum_j Bm[a][j] V[j][b]
+    double bv[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) bv[a][b] = Bm[a][0] * f.V[0 * 3 + b] + Bm[a][1] * f.V[1 * 3 + b] + Bm[a][2] * f.V[2 * 3 + b];
+    // M in the (ka,kb) pair basis: diagonal pairs via S, off pairs via B blocks
+    // B block k couples idx (i,j) and (j,i) for (i,j) = (0,1), (1,2), (2,0)
+    const int PI[3] = {0, 1, 2}, PJ[3] = {1, 2, 0};
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        // Y[i][l] for l = ka*3+kb: sum_k z_k[(a,i)] M[k][l]
+        double Y[3][9];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+#pragma unroll
+            for (int l = 0; l < 9; l++) Y[i][l] = 0.0;
+#pragma unroll
+            for (int be = 0; be < 3; be++) {
+                double s = 0.0;
+#pragma unroll
+                for (int al = 0; al < 3; al++) s += bv[a][al] * f.U[i * 3 + al] * f.S[al][be];
+                Y[i][be * 4] = s * sc;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int pi = PI[k], pj = PJ[k];
+                // z_(pi,pj)[(a,i)] = bv[a][pj] U[i][pi]; z_(pj,pi)[(a,i)] = bv[a][pi] U[i][pj]
+                double zij = bv[a][pj] * f.U[i * 3 + pi], zji = bv[a][pi] * f.U[i * 3 + pj];
+                // B block in row-major [[B00, B01], [B10, B11]] on basis ((pi,pj), (pj,pi)),
+                // except pair (2,0) whose block is stored transposed (neo_hookean.py:120-123)
+                double B00, B01, B10, B11;
+                if (k < 2) { B00 = f.B[k][0]; B01 = f.B[k][1]; B10 = f.B[k][2]; B11 = f.B[k][3]; }
+                else { B00 = f.B[k][3]; B01 = f.B[k][2]; B10 = f.B[k][1]; B11 = f.B[k][0]; }
+                Y[i][pi * 3 + pj] = (zij * B00 + zji * B10) * sc;
+                Y[i][pj * 3 + pi] = (zij * B01 + zji * B11) * sc;
+            }
+        }
+#pragma unroll
+        for (int c = a; c < 4; c++) {
+            double blk[9];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int l = 0; l < 3; l++) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int ka = 0; ka < 3; ka++)
+#pragma unroll
+                        for (int kb = 0; kb < 3; kb++) s += Y[i][ka * 3 + kb] * bv[c][kb] * f.U[l * 3 + ka];
+                    blk[i * 3 + l] = s;
+                }
+            emit(a, c, blk);
+        }
+    }
+}
+
+__device__ __forceinline__ void tet_load(const int* tets, const double* IB, const double* x, int e, int (&t)[4],
+                                         double (&F)[9], double (&Bm)[4][3])
+{
+    for (int a = 0; a < 4; a++) t[a] = tets[4 * e + a];
+    double ib[9];
+    for (int k = 0; k < 9; k++) ib[k] = IB[9 * e + k];
+    v3 x0 = ldv(x, t[0]);
+    v3 d1 = ldv(x, t[1]) - x0, d2 = ldv(x, t[2]) - x0, d3 = ldv(x, t[3]) - x0;
+    double Ds[9] = {d1.x, d2.x, d3.x, d1.y, d2.y, d3.y, d1.z, d2.z, d3.z};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) F[i * 3 + j] = Ds[i * 3] * ib[j] + Ds[i * 3 + 1] * ib[3 + j] + Ds[i * 3 + 2] * ib[6 + j];
+    for (int j = 0; j < 3; j++) {
+        Bm[1][j] = ib[j];
+        Bm[2][j] = ib[3 + j];
+        Bm[3][j] = ib[6 + j];
+        Bm[0][j] = -ib[j] - ib[3 + j] - ib[6 + j];
+    }
+}
+
+// compute_elasticity (NEWTON_IPC.py:568-691): one thread per tet
+__global__ void __launch_bounds__(TPB) k_elastic(int nt, int nnzb, const int* __restrict__ tets, const double* __restrict__ IB,
+                                                 const double* __restrict__ vol, const double* __restrict__ la,
+                                                 const double* __restrict__ mu, const double* __restrict__ x,
+                                                 const int* __restrict__ slots, double dt2, double* val, double* rhs)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nt) return;
+    int t[4];
+    double F[9], Bm[4][3];
+    tet_load(tets, IB, x, e, t, F, Bm);
+    double sc = dt2 * vol[e];
+    double lae = la[e], mue = mu[e];
+    double P[9];
+    nh_P(F, lae, mue, P);
+    for (int a = 0; a < 4; a++)
+        for (int i = 0; i < 3; i++) {
+            double gi = (P[i * 3] * Bm[a][0] + P[i * 3 + 1] * Bm[a][1] + P[i * 3 + 2] * Bm[a][2]) * sc;
+            atomicAdd(&rhs[3 * t[a] + i], -gi);
+        }
+    NHFactor f;
+    nh_factor(F, lae, mue, f);
+    const int* sl = slots + 16 * e;
+    tet_hessian_blocks(f, Bm, sc, [&](int a, int c, const double* blk) {
+        int b = sl[a * 4 + c];
+#pragma unroll
+        for (int k = 0; k < 9; k++) atomicAdd(&val[k * nnzb + b], blk[k]);
+        if (c != a) {
+            int bt = sl[c * 4 + a];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int l = 0; l < 3; l++) atomicAdd(&val[(l * 3 + i) * nnzb + bt], blk[i * 3 + l]);
+        }
+    });
+}
+
+// contact rows: a group of G lanes per row (G = 2 for PP, 4 otherwise).
+// Every lane evaluates the constraint and runs the Jacobi sweeps on its own
+// copy of the packed Hessian; lane j owns eigenvector rows 3j..3j+2 and
+// writes rows 3j..3j+2 of the projected Hessian (makePD semantics, UTILS.h:7-25).
+template <int KIND>
+struct KindInfo {
+    static constexpr int K = kind_points(KIND);
+    static constexpr int N = 3 * K;
+    static constexpr int G = K == 2 ? 2 : 4;
+};
+
+template <int KIND>
+__device__ __forceinline__ void load_row(const int* rows, int r, const double* x, const double* x0, int (&id)[4], v3 (&px)[4], v3 (&p0)[4])
+{
+    constexpr int K = KindInfo<KIND>::K;
+    constexpr int W = KIND == 0 ? 2 : (KIND == 1 ? 3 : 4);
+#pragma unroll
+    for (int a = 0; a < 4; a++) id[a] = a < K ? rows[W * r + a] : 0;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        px[a] = a < K ? ldv(x, id[a]) : mk(0, 0, 0);
+        p0[a] = (KIND >= 4 && a < K) ? ldv(x0, id[a]) : mk(0, 0, 0);
+    }
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(TPB) k_contact(int n, const int* __restrict__ rows, const double* __restrict__ x,
+                                                 const double* __restrict__ x0, double dHat2, double kappa,
+                                                 double* __restrict__ Hout, double* rhs, double* cdiag,
+                                                 double* Eout, double* gout, int project, int accumulate)
+{
+    using KI = KindInfo<KIND>;
+    constexpr int N = KI::N, G = KI::G, K = KI::K;
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = gid / G, lane = gid % G;
+    if (r >= n) return;
+    int id[4];
+    v3 px[4], p0[4];
+    load_row<KIND>(rows, r, x, x0, id, px, p0);
+    double g[N], H[N * N];
+    double E = constraint_gH<KIND>(px, p0, dHat2, kappa, g, H);
+    double A[N * (N + 1) / 2];
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = i; j < N; j++) A[pk(N, i, j)] = H[i * N + j];
+    double V[3][N];
+    double out[3][N];
+    if (project) {
+        jacobi_eig<N, 3>(A, V, lane * 3, 20);
+        double lam[N];
+#pragma unroll
+        for (int k = 0; k < N; k++) lam[k] = fmax(A[pk(N, k, k)], 0.0);
+        double lv[3][N];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int k = 0; k < N; k++) { lv[i][k] = lam[k] * V[i][k]; out[i][k] = 0.0; }
+        // rows c of V live in lane c / 3 of this group
+        int base = (threadIdx.x / G) * G;
+#pragma unroll
+        for (int c = 0; c < N; c++) {
+            double vc[N];
+#pragma unroll
+            for (int k = 0; k < N; k++) vc[k] = __shfl(V[c % 3][k], base + c / 3, 64);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < N; k++) s += lv[i][k] * vc[k];
+                out[i][c] = s;
+            }
+        }
+    } else {
+        // rows of the unprojected Hessian owned by this lane (static indices)
+#pragma unroll
+        for (int L = 0; L < K; L++)
+            if (L == lane) {
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int c = 0; c < N; c++) out[i][c] = H[(L * 3 + i) * N + c];
+            }
+    }
+    if (lane >= K) return;
+    double* Hr = Hout + (size_t)r * N * N + lane * 3 * N;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int c = 0; c < N; c++) Hr[i * N + c] = out[i][c];
+#pragma unroll
+    for (int L = 0; L < K; L++) {
+        if (L != lane) continue;
+        if (accumulate) {
+            int v = id[L];
+            // load_hessian_and_gradient (NEWTON_IPC.py:545-555): rhs -= g
+#pragma unroll
+            for (int i = 0; i < 3; i++) atomicAdd(&rhs[3 * v + i], -g[L * 3 + i]);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) atomicAdd(&cdiag[9 * v + i * 3 + j], out[i][L * 3 + j]);
+        } else {
+            if (L == 0 && Eout) Eout[r] = E;
+            if (gout) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) gout[(size_t)r * N + L * 3 + i] = g[L * 3 + i];
+            }
+        }
+    }
+}
+
+// block-Jacobi preconditioner: inverse of (elastic diag block + contact diag
+// block); Dirichlet vertices get the identity (their residual is zero)
+__global__ void k_precond(int nv, int nnzb, const double* __restrict__ val, const int* __restrict__ diag,
+                          const double* __restrict__ cdiag, const unsigned char* __restrict__ dmask, double* Minv)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    double A[9];
+    int b = diag[i];
+    for (int k = 0; k < 9; k++) A[k] = val[k * nnzb + b] + cdiag[9 * i + k];
+    double* o = Minv + 9 * i;
+    double d = det3(A);
+    if (dmask[i] || !(fabs(d) > 0.0)) {
+        for (int k = 0; k < 9; k++) o[k] = (k % 4 == 0) ? (dmask[i] ? 1.0 : 1.0 / fmax(fabs(A[k]), 1e-300)) : 0.0;
+        return;
+    }
+    double id = 1.0 / d;
+    o[0] = (A[4] * A[8] - A[5] * A[7]) * id; o[1] = (A[2] * A[7] - A[1] * A[8]) * id; o[2] = (A[1] * A[5] - A[2] * A[4]) * id;
+    o[3] = (A[5] * A[6] - A[3] * A[8]) * id; o[4] = (A[0] * A[8] - A[2] * A[6]) * id; o[5] = (A[2] * A[3] - A[0] * A[5]) * id;
+    o[6] = (A[3] * A[7] - A[4] * A[6]) * id; o[7] = (A[1] * A[6] - A[0] * A[7]) * id; o[8] = (A[0] * A[4] - A[1] * A[3]) * id;
+}
+
+// ---------------------------------------------------------------------------
+// SpMV: q += A p  (BSR part) and q += sum_rows H_r p_r (contact part)
+// ---------------------------------------------------------------------------
+#define SPMV_G 8
+// 8 lanes per block row; lanes stride over the row's blocks, then reduce.
+// Also writes per-block partials of p.q (after Dirichlet masking) when part != 0.
+template <class T>
+__global__ void __launch_bounds__(TPB) k_bsr_spmv(int nv, int nnzb, const int* __restrict__ rowptr,
+                                                  const int* __restrict__ col, const T* __restrict__ val,
+                                                  const double* __restrict__ p, double* __restrict__ q,
+                                                  const unsigned char* __restrict__ dmask, double* part,
+                                                  const int* __restrict__ state, int it)
+{
+    if (state && it > state[0]) return;
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    int row = gid / SPMV_G, lane = gid % SPMV_G;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if (row < nv) {
+        int b0 = rowptr[row], b1 = rowptr[row + 1];
+        for (int b = b0 + lane; b < b1; b += SPMV_G) {
+            int c = col[b];
+            double x0 = p[3 * c], x1 = p[3 * c + 1], x2 = p[3 * c + 2];
+            a0 += (double)val[0 * nnzb + b] * x0 + (double)val[1 * nnzb + b] * x1 + (double)val[2 * nnzb + b] * x2;
+            a1 += (double)val[3 * nnzb + b] * x0 + (double)val[4 * nnzb + b] * x1 + (double)val[5 * nnzb + b] * x2;
+            a2 += (double)val[6 * nnzb + b] * x0 + (double)val[7 * nnzb + b] * x1 + (double)val[8 * nnzb + b] * x2;
+        }
+    }
+#pragma unroll
+    for (int o = SPMV_G / 2; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, 64);
+        a1 += __shfl_xor(a1, o, 64);
+        a2 += __shfl_xor(a2, o, 64);
+    }
+    double pq = 0.0;
+    if (row < nv && lane == 0) {
+        double y0 = q[3 * row] + a0, y1 = q[3 * row + 1] + a1, y2 = q[3 * row + 2] + a2;
+        if (dmask[row]) y0 = y1 = y2 = 0.0;
+        q[3 * row] = y0; q[3 * row + 1] = y1; q[3 * row + 2] = y2;
+        pq = p[3 * row] * y0 + p[3 * row + 1] * y1 + p[3 * row + 2] * y2;
+    }
+    if (part) {
+        pq = block_sum(pq);
+        if (threadIdx.x == 0) part[blockIdx.x] = pq;
+    }
+}
+
+template <int KIND>
+__global__ void k_contact_spmv(int n, const int* __restrict__ rows, const double* __restrict__ Hc,
+                               const double* __restrict__ p, double* q, const int* __restrict__ state, int it)
+{
+    if (state && it > state[0]) return;
+    using KI = KindInfo<KIND>;
+    constexpr int N = KI::N, K = KI::K;
+    constexpr int W = KIND == 0 ? 2 : (KIND == 1 ? 3 : 4);
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = gid / K, lane = gid % K;
+    if (r >= n) return;
+    double pv[N];
+#pragma unroll
+    for (int a = 0; a < K; a++) {
+        int v = rows[W * r + a];
+        pv[3 * a] = p[3 * v]; pv[3 * a + 1] = p[3 * v + 1]; pv[3 * a + 2] = p[3 * v + 2];
+    }
+    const double* Hr = Hc + (size_t)r * N * N + lane * 3 * N;
+    int v = rows[W * r + lane];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < N; c++) s += Hr[i * N + c] * pv[c];
+        atomicAdd(&q[3 * v + i], s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PCG with block-Jacobi preconditioner (replaces the CHOLMOD solve of
+// NEWTON_IPC.py:773-775).  Dirichlet dofs carry a prescribed increment in the
+// initial guess and a zero residual afterwards.
+// ---------------------------------------------------------------------------
+// x = p0 (prescribed on Dirichlet dofs, 0 elsewhere)
+__global__ void k_pcg_x0(int nv, const double* xt, const double* xcur, const unsigned char* dmask, double* sol, double* q)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    for (int d = 0; d < 3; d++) {
+        sol[3 * i + d] = dmask[i] ? xt[3 * i + d] - xcur[3 * i + d] : 0.0;
+        q[3 * i + d] = 0.0;
+    }
+}
+// r = b - q (q = A x0, zero when x0 = 0); z = Minv r; p = z; partials of r.z and b.b
+__global__ void k_pcg_init(int nv, const double* b, const double* q, const unsigned char* dmask, const double* Minv,
+                           double* r, double* z, double* p, double* qz, double* part_rz, double* part_bb)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, bb = 0.0;
+    if (i < nv) {
+        double ri[3], zi[3];
+        bool fixed = dmask[i];
+        for (int d = 0; d < 3; d++) ri[d] = fixed ? 0.0 : b[3 * i + d] - q[3 * i + d];
+        const double* M = Minv + 9 * i;
+        for (int d = 0; d < 3; d++) zi[d] = fixed ? 0.0 : M[3 * d] * ri[0] + M[3 * d + 1] * ri[1] + M[3 * d + 2] * ri[2];
+        for (int d = 0; d < 3; d++) {
+            r[3 * i + d] = ri[d]; z[3 * i + d] = zi[d]; p[3 * i + d] = zi[d]; qz[3 * i + d] = 0.0;
+            rz += ri[d] * zi[d];
+            bb += ri[d] * ri[d];
+        }
+    }
+    rz = block_sum(rz);
+    bb = block_sum(bb);
+    if (threadIdx.x == 0) { part_rz[blockIdx.x] = rz; part_bb[blockIdx.x] = bb; }
+}
+// scal[0] = rz, scal[2] = bb from partials
+__global__ void k_pcg_scalars(const double* part_rz, const double* part_bb, int n, double* scal, int* state)
+{
+    double rz = reduce_partials(part_rz, n);
+    double bb = reduce_partials(part_bb, n);
+    if (threadIdx.x == 0) {
+        scal[0] = rz; scal[1] = rz; scal[2] = bb; scal[3] = bb;
+        state[0] = bb == 0.0 ? -1 : 0x7fffffff;  // done_at
+        state[1] = 0;
+    }
+}
+// alpha = rz / pq; x += alpha p; r -= alpha q; z = Minv r; partials of r.z, r.r
+__global__ void k_pcg_upd1(int nv, const double* part_pq, int npq, const double* scal, const unsigned char* dmask,
+                           const double* Minv, const double* p, const double* q, double* x, double* r, double* z,
+                           double* part_rz, double* part_rr, const int* state, int it)
+{
+    if (it > state[0]) return;
+    double pq = reduce_partials(part_pq, npq);
+    double alpha = scal[it & 1] / pq;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (i < nv && !dmask[i]) {
+        double ri[3];
+        for (int d = 0; d < 3; d++) {
+            x[3 * i + d] += alpha * p[3 * i + d];
+            ri[d] = r[3 * i + d] - alpha * q[3 * i + d];
+            r[3 * i + d] = ri[d];
+        }
+        const double* M = Minv + 9 * i;
+        for (int d = 0; d < 3; d++) {
+            double zd = M[3 * d] * ri[0] + M[3 * d + 1] * ri[1] + M[3 * d + 2] * ri[2];
+            z[3 * i + d] = zd;
+            rz += ri[d] * zd;
+            rr += ri[d] * ri[d];
+        }
+    }
+    rz = block_sum(rz);
+    rr = block_sum(rr);
+    if (threadIdx.x == 0) { part_rz[blockIdx.x] = rz; part_rr[blockIdx.x] = rr; }
+}
+// beta = rz_new / rz; p = z + beta p; q = 0; convergence test on |r|^2 <= tol^2 |b|^2
+__global__ void k_pcg_upd2(int nv, const double* part_rz, const double* part_rr, int nu, double* scal,
+                           const double* z, double* p, double* q, int* state, int it, double tol2)
+{
+    if (it > state[0]) return;
+    double rzn = reduce_partials(part_rz, nu);
+    double rr = reduce_partials(part_rr, nu);
+    double beta = rzn / scal[it & 1];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3 * nv) {
+        p[i] = z[i] + beta * p[i];
+        q[i] = 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[(it + 1) & 1] = rzn;
+        scal[4] = rr;
+        state[1] = it + 1;
+        if (rr <= tol2 * scal[2] || !(rr == rr)) state[0] = it;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// energy (NEWTON_IPC.py:514-542): per-block partial sums, summed in order
+// ---------------------------------------------------------------------------
+__global__ void k_energy_inertia(int nv, const double* m, const double* x, const double* xt, double* part)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (i < nv) {
+        v3 d = ldv(x, i) - ldv(xt, i);
+        e = 0.5 * m[i] * n2(d);
+    }
+    e = block_sum(e);
+    if (threadIdx.x == 0) part[blockIdx.x] = e;
+}
+__global__ void k_energy_elastic(int nt, const int* tets, const double* IB, const double* vol, const double* la,
+                                 const double* mu, const double* x, double dt2, double* part)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double en = 0.0;
+    if (e < nt) {
+        int t[4];
+        double F[9], Bm[4][3];
+        tet_load(tets, IB, x, e, t, F, Bm);
+        en = nh_energy_F(F, la[e], mu[e]) * dt2 * vol[e];
+    }
+    en = block_sum(en);
+    if (threadIdx.x == 0) part[blockIdx.x] = en;
+}
+template <int KIND>
+__global__ void k_energy_contact(int n, const int* rows, const double* x, const double* x0, double dHat2, double kappa,
+                                 double* part)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    double en = 0.0;
+    if (r < n) {
+        int id[4];
+        v3 px[4], p0[4];
+        load_row<KIND>(rows, r, x, x0, id, px, p0);
+        en = constraint_E(KIND, px, p0, dHat2, kappa);
+    }
+    en = block_sum(en);
+    if (threadIdx.x == 0) part[blockIdx.x] = en;
+}
+template <int KIND>
+__global__ void k_min_dist(int n, const int* rows, const double* x, unsigned long long* out)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int id[4];
+    v3 px[4], p0[4];
+    load_row<KIND>(rows, r, x, x, id, px, p0);
+    double d = constraint_dist(KIND, px);
+    atomicMin(out, (unsigned long long)__double_as_longlong(d));
+}
+
+// inversion filter (NEWTON_IPC.py:443-453)
+__global__ void k_inversion(int nt, const int* tets, const double* x, const double* p, unsigned long long* amin)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nt) return;
+    int a = tets[4 * e], b = tets[4 * e + 1], c = tets[4 * e + 2], d = tets[4 * e + 3];
+    double al = tet_inversion_alpha(ldv(x, a), ldv(x, b), ldv(x, c), ldv(x, d), ldv(p, a), ldv(p, b), ldv(p, c), ldv(p, d), 0.2);
+    atomicMin(amin, (unsigned long long)__double_as_longlong(al));
+}
+
+// batched tet evaluation for tests
+__global__ void k_batch_tet(int n, const double* Fin, const double* IB, const double* scale, const double* la,
+                            const double* mu, double* psi, double* P, double* Hout)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double F[9], ib[9];
+    for (int k = 0; k < 9; k++) { F[k] = Fin[9 * e + k]; ib[k] = IB[9 * e + k]; }
+    psi[e] = nh_energy_F(F, la[e], mu[e]);
+    double Pl[9];
+    nh_P(F, la[e], mu[e], Pl);
+    for (int k = 0; k < 9; k++) P[9 * e + k] = Pl[k];
+    if (!Hout) return;
+    double Bm[4][3];
+    for (int j = 0; j < 3; j++) {
+        Bm[1][j] = ib[j]; Bm[2][j] = ib[3 + j]; Bm[3][j] = ib[6 + j];
+        Bm[0][j] = -ib[j] - ib[3 + j] - ib[6 + j];
+    }
+    NHFactor f;
+    nh_factor(F, la[e], mu[e], f);
+    double* H = Hout + 144 * (size_t)e;
+    tet_hessian_blocks(f, Bm, scale[e], [&](int a, int c, const double* blk) {
+        for (int i = 0; i < 3; i++)
+            for (int l = 0; l < 3; l++) {
+                H[(3 * a + i) * 12 + 3 * c + l] = blk[i * 3 + l];
+                H[(3 * c + l) * 12 + 3 * a + i] = blk[i * 3 + l];
+            }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int tm_begin(fipc_ctx* c, KernelTimer& t)
+{
+    if (!c->timing) return FIPC_OK;
+    if (!t.a) { HIPCHK(hipEventCreate(&t.a)); HIPCHK(hipEventCreate(&t.b)); }
+    HIPCHK(hipEventRecord(t.a, c->s));
+    return FIPC_OK;
+}
+static int tm_end(fipc_ctx* c, KernelTimer& t)
+{
+    if (!c->timing) return FIPC_OK;
+    HIPCHK(hipEventRecord(t.b, c->s));
+    HIPCHK(hipEventSynchronize(t.b));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
+    t.total_ms += ms;
+    t.launches++;
+    return FIPC_OK;
+}
+
+template <class T>
+static int upload(DVec<T>& d, const T* h, size_t n, hipStream_t s)
+{
+    STCHK(ensure(d, n));
+    if (n) HIPCHK(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return FIPC_OK;
+}
+
+extern "C" int fipc_create(fipc_ctx** out)
+{
+    if (!out) return FIPC_ERR_ARG;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev <= 0) return FIPC_ERR_HIP;
+    fipc_ctx* c = new fipc_ctx();
+    HIPCHK(hipStreamCreate(&c->s));
+    *out = c;
+    return FIPC_OK;
+}
+
+extern "C" int fipc_destroy(fipc_ctx* c)
+{
+    if (!c) return FIPC_OK;
+    (void)hipStreamSynchronize(c->s);
+    release(c->tets); release(c->bp); release(c->be); release(c->bt); release(c->dids);
+    release(c->IB); release(c->vol); release(c->la); release(c->mu); release(c->m); release(c->dmask);
+    release(c->x); release(c->x0); release(c->xn); release(c->xt); release(c->xprev); release(c->v);
+    release(c->sol); release(c->rhs); release(c->target);
+    release(c->rowptr); release(c->col); release(c->diag); release(c->slots); release(c->val); release(c->valf);
+    for (int t = 0; t < NTYPES; t++) { release(c->rows[t]); release(c->cH[t]); }
+    release(c->keys_pp); release(c->keys_pe); release(c->keys_tmp); release(c->counters); release(c->cub_tmp);
+    release(c->nuniq); release(c->cdiag); release(c->Minv); release(c->r); release(c->z); release(c->pp); release(c->q);
+    release(c->part_a); release(c->part_b); release(c->part_c); release(c->scal); release(c->pcg_state);
+    release(c->epart); release(c->amin); release(c->flag);
+    release(c->grid.cnt); release(c->grid.ekey); release(c->grid.eitem); release(c->grid.bstart);
+    release(c->grid.bfill); release(c->grid.items);
+    KernelTimer* ts[] = {&c->tm_spmv, &c->tm_assemble, &c->tm_ccd, &c->tm_find, &c->tm_energy, &c->tm_contact};
+    for (auto* t : ts) { if (t->a) { (void)hipEventDestroy(t->a); (void)hipEventDestroy(t->b); } }
+    (void)hipStreamDestroy(c->s);
+    delete c;
+    return FIPC_OK;
+}
+
+// BSR pattern of the elastic + inertia Hessian: one block per vertex pair
+// sharing a tet (and every diagonal block).  slots[e*16 + a*4 + c] is the
+// block of (tet vertex a, tet vertex c).
+static int build_pattern(fipc_ctx* c, int nv, int nt, const int* tets)
+{
+    std::vector<std::vector<int>> adj(nv);
+    for (int i = 0; i < nv; i++) adj[i].push_back(i);
+    for (int e = 0; e < nt; e++)
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) adj[tets[4 * e + a]].push_back(tets[4 * e + b]);
+    std::vector<int> rowptr(nv + 1, 0), diag(nv);
+    for (int i = 0; i < nv; i++) {
+        auto& v = adj[i];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        rowptr[i + 1] = rowptr[i] + (int)v.size();
+    }
+    int nnzb = rowptr[nv];
+    std::vector<int> col(nnzb);
+    for (int i = 0; i < nv; i++) {
+        std::copy(adj[i].begin(), adj[i].end(), col.begin() + rowptr[i]);
+        diag[i] = rowptr[i] + (int)(std::lower_bound(adj[i].begin(), adj[i].end(), i) - adj[i].begin());
+    }
+    std::vector<int> slots((size_t)nt * 16);
+    for (int e = 0; e < nt; e++)
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                int i = tets[4 * e + a], j = tets[4 * e + b];
+                slots[(size_t)e * 16 + a * 4 + b] = rowptr[i] + (int)(std::lower_bound(adj[i].begin(), adj[i].end(), j) - adj[i].begin());
+            }
+    c->nnzb = nnzb;
+    STCHK(upload(c->rowptr, rowptr.data(), rowptr.size(), c->s));
+    STCHK(upload(c->col, col.data(), col.size(), c->s));
+    STCHK(upload(c->diag, diag.data(), diag.size(), c->s));
+    STCHK(upload(c->slots, slots.data(), slots.size(), c->s));
+    STCHK(ensure(c->val, (size_t)nnzb * 9));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return FIPC_OK;
+}
+
+extern "C" int fipc_upload_mesh(fipc_ctx* c, int nv, const double* x0, const double* m, int nt, const int* tets,
+                                const double* IB, const double* vol, const double* la, const double* mu, int nbp,
+                                const int* bp, int nbe, const int* be, int nbt, const int* bt,
+                                const unsigned char* dirichlet)
+{
+    if (!c || nv <= 0 || nt < 0) return FIPC_ERR_ARG;
+    if (nv >= (1 << 21)) return FIPC_ERR_CAPACITY;
+    c->nv = nv; c->nt = nt; c->nbp = nbp; c->nbe = nbe; c->nbt = nbt;
+    STCHK(upload(c->x0, x0, (size_t)nv * 3, c->s));
+    STCHK(upload(c->m, m, nv, c->s));
+    STCHK(upload(c->tets, tets, (size_t)nt * 4, c->s));
+    STCHK(upload(c->IB, IB, (size_t)nt * 9, c->s));
+    STCHK(upload(c->vol, vol, nt, c->s));
+    STCHK(upload(c->la, la, nt, c->s));
+    STCHK(upload(c->mu, mu, nt, c->s));
+    STCHK(upload(c->bp, bp, nbp, c->s));
+    STCHK(upload(c->be, be, (size_t)nbe * 2, c->s));
+    STCHK(upload(c->bt, bt, (size_t)nbt * 3, c->s));
+    STCHK(upload(c->dmask, dirichlet, nv, c->s));
+    size_t n3 = (size_t)nv * 3;
+    STCHK(ensure(c->x, n3)); STCHK(ensure(c->xn, n3)); STCHK(ensure(c->xt, n3)); STCHK(ensure(c->xprev, n3));
+    STCHK(ensure(c->v, n3)); STCHK(ensure(c->sol, n3)); STCHK(ensure(c->rhs, n3)); STCHK(ensure(c->target, n3));
+    STCHK(ensure(c->r, n3)); STCHK(ensure(c->z, n3)); STCHK(ensure(c->pp, n3)); STCHK(ensure(c->q, n3));
+    STCHK(ensure(c->cdiag, (size_t)nv * 9)); STCHK(ensure(c->Minv, (size_t)nv * 9));
+    HIPCHK(hipMemcpyAsync(c->x.p, x0, n3 * 8, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->xt.p, x0, n3 * 8, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemsetAsync(c->v.p, 0, n3 * 8, c->s));
+    HIPCHK(hipMemsetAsync(c->sol.p, 0, n3 * 8, c->s));
+    HIPCHK(hipMemsetAsync(c->rhs.p, 0, n3 * 8, c->s));
+    int nmax = std::max(std::max(nv, nt), 4096);
+    STCHK(ensure(c->part_a, nblk(nmax) + 64)); STCHK(ensure(c->part_b, nblk(nmax) + 64));
+    STCHK(ensure(c->part_c, nblk(nmax) + 64));
+    STCHK(ensure(c->scal, 8)); STCHK(ensure(c->pcg_state, 4));
+    STCHK(ensure(c->epart, nblk(nmax) * 4 + 64));
+    STCHK(ensure(c->amin, 4)); STCHK(ensure(c->flag, 4)); STCHK(ensure(c->counters, 16)); STCHK(ensure(c->nuniq, 4));
+    for (int t = 0; t < NTYPES; t++) {
+        if (c->cap[t] == 0) c->cap[t] = 1024;
+        int w = t == 0 ? 2 : (t == 1 ? 3 : 4);
+        int N = 3 * (t == 0 ? 2 : (t == 1 ? 3 : 4));
+        STCHK(ensure(c->rows[t], c->cap[t] * w));
+        STCHK(ensure(c->cH[t], c->cap[t] * N * N));
+        c->count[t] = 0;
+    }
+    return build_pattern(c, nv, nt, tets);
+}
+
+extern "C" int fipc_set_params(fipc_ctx* c, double dt, double dHat, double kappa, double inv_dx, double gravity)
+{
+    if (!c || !(dt > 0) || !(dHat > 0) || !(inv_dx > 0)) return FIPC_ERR_ARG;
+    c->dt = dt; c->dHat = dHat; c->dHat2 = dHat * dHat; c->kappa = kappa; c->inv_dx = inv_dx; c->gravity = gravity;
+    return FIPC_OK;
+}
+
+extern "C" int fipc_set_state(fipc_ctx* c, const double* x, const double* v)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    size_t n = (size_t)c->nv * 3 * 8;
+    if (x) HIPCHK(hipMemcpyAsync(c->x.p, x, n, hipMemcpyHostToDevice, c->s));
+    if (v) HIPCHK(hipMemcpyAsync(c->v.p, v, n, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return FIPC_OK;
+}
+extern "C" int fipc_get_state(fipc_ctx* c, double* x, double* v)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    size_t n = (size_t)c->nv * 3 * 8;
+    if (x) HIPCHK(hipMemcpyAsync(x, c->x.p, n, hipMemcpyDeviceToHost, c->s));
+    if (v) HIPCHK(hipMemcpyAsync(v, c->v.p, n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return FIPC_OK;
+}
+static DVec<double>* vec_of(fipc_ctx* c, int which)
+{
+    switch (which) {
+    case 0: return &c->x;
+    case 1: return &c->sol;
+    case 2: return &c->rhs;
+    case 3: return &c->xt;
+    case 4: return &c->v;
+    case 5: return &c->xprev;
+    default: return nullptr;
+    }
+}
+extern "C" int fipc_get_vector(fipc_ctx* c, int which, double* out)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    DVec<double>* d = vec_of(c, which);
+    if (!d || !out) return FIPC_ERR_ARG;
+    HIPCHK(hipMemcpyAsync(out, d->p, (size_t)c->nv * 24, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return FIPC_OK;
+}
+extern "C" int fipc_set_vector(fipc_ctx* c, int which, const double* in)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    DVec<double>* d = vec_of(c, which);
+    if (!d || !in) return FIPC_ERR_ARG;
+    HIPCHK(hipMemcpyAsync(d->p, in, (size_t)c->nv * 24, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return FIPC_OK;
+}
+
+extern "C" int fipc_begin_step(fipc_ctx* c, const double* target)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    if (target) HIPCHK(hipMemcpyAsync(c->target.p, target, (size_t)c->nv * 24, hipMemcpyHostToDevice, c->s));
+    else HIPCHK(hipMemcpyAsync(c->target.p, c->x.p, (size_t)c->nv * 24, hipMemcpyDeviceToDevice, c->s));
+    k_begin_step<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->x.p, c->v.p, c->xn.p, c->xt.p, c->dmask.p, c->target.p, c->dt, c->gravity);
+    HIPCHK(hipGetLastError());
+    return FIPC_OK;
+}
+extern "C" int fipc_end_step(fipc_ctx* c)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    k_end_step<<<nblk(c->nv * 3), TPB, 0, c->s>>>(c->nv * 3, c->x.p, c->xn.p, c->v.p, 1.0 / c->dt);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->s));
+    return FIPC_OK;
+}
+extern "C" int fipc_save_prev(fipc_ctx* c)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    HIPCHK(hipMemcpyAsync(c->xprev.p, c->x.p, (size_t)c->nv * 24, hipMemcpyDeviceToDevice, c->s));
+    return FIPC_OK;
+}
+extern "C" int fipc_apply_sol(fipc_ctx* c, double alpha)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    k_apply<<<nblk(c->nv * 3), TPB, 0, c->s>>>(c->nv * 3, c->xprev.p, c->sol.p, c->x.p, alpha);
+    HIPCHK(hipGetLastError());
+    return FIPC_OK;
+}
+extern "C" int fipc_residual(fipc_ctx* c, double* res)
+{
+    if (!c || !c->nv || !res) return FIPC_ERR_STATE;
+    int nb = std::min(nblk(c->nv * 3), 1024);
+    k_maxabs<<<nb, TPB, 0, c->s>>>(c->nv * 3, c->sol.p, c->part_a.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(nb);
+    HIPCHK(hipMemcpyAsync(h.data(), c->part_a.p, nb * 8, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    double m = 0.0;
+    for (double v : h) m = std::max(m, v);
+    *res = m;
+    return FIPC_OK;
+}
+
+// ---- assembly ----
+static int launch_contacts(fipc_ctx* c, int accumulate)
+{
+    for (int t = 0; t < NTYPES; t++) {
+        int n = c->count[t];
+        if (!n) continue;
+        int G = t == 0 ? 2 : 4;
+        int nb = nblk((long)n * G);
+        double* H = c->cH[t].p;
+#define LAUNCH_C(K) k_contact<K><<<nb, TPB, 0, c->s>>>(n, c->rows[t].p, c->x.p, c->x0.p, c->dHat2, c->kappa, H, c->rhs.p, c->cdiag.p, nullptr, nullptr, 1, accumulate)
+        switch (t) {
+        case 0: LAUNCH_C(0); break;
+        case 1: LAUNCH_C(1); break;
+        case 2: LAUNCH_C(2); break;
+        case 3: LAUNCH_C(3); break;
+        case 4: LAUNCH_C(4); break;
+        case 5: LAUNCH_C(5); break;
+        default: LAUNCH_C(6); break;
+        }
+#undef LAUNCH_C
+        HIPCHK(hipGetLastError());
+    }
+    return FIPC_OK;
+}
+
+extern "C" int fipc_assemble(fipc_ctx* c, int precision)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    c->precision = precision;
+    STCHK(tm_begin(c, c->tm_assemble));
+    HIPCHK(hipMemsetAsync(c->val.p, 0, (size_t)c->nnzb * 9 * 8, c->s));
+    k_inertia<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->nnzb, c->m.p, c->x.p, c->xt.p, c->diag.p, c->val.p, c->rhs.p, c->cdiag.p);
+    HIPCHK(hipGetLastError());
+    if (c->nt)
+        k_elastic<<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->nnzb, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p, c->x.p,
+                                                  c->slots.p, c->dt * c->dt, c->val.p, c->rhs.p);
+    HIPCHK(hipGetLastError());
+    STCHK(tm_end(c, c->tm_assemble));
+    STCHK(tm_begin(c, c->tm_contact));
+    STCHK(launch_contacts(c, 1));
+    STCHK(tm_end(c, c->tm_contact));
+    k_precond<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->nnzb, c->val.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p);
+    HIPCHK(hipGetLastError());
+    if (precision == 1) {
+        STCHK(ensure(c->valf, (size_t)c->nnzb * 9));
+        // FP32 copy of the BSR values for the SpMV
+        extern int fipc_convert_f32(fipc_ctx*);
+        STCHK(fipc_convert_f32(c));
+    }
+    return FIPC_OK;
+}
+
+__global__ void k_to_f32(size_t n, const double* a, float* b)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+int fipc_convert_f32(fipc_ctx* c)
+{
+    size_t n = (size_t)c->nnzb * 9;
+    k_to_f32<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, c->s>>>(n, c->val.p, c->valf.p);
+    HIPCHK(hipGetLastError());
+    return FIPC_OK;
+}
+
+// ---- SpMV ----
+static int spmv(fipc_ctx* c, const double* p, double* q, double* part, const int* state, int it)
+{
+    STCHK(tm_begin(c, c->tm_spmv));
+    for (int t = 0; t < NTYPES; t++) {
+        int n = c->count[t];
+        if (!n) continue;
+        int K = t == 0 ? 2 : (t == 1 ? 3 : 4);
+        int nb = nblk((long)n * K);
+#define LAUNCH_S(KK) k_contact_spmv<KK><<<nb, TPB, 0, c->s>>>(n, c->rows[t].p, c->cH[t].p, p, q, state, it)
+        switch (t) {
+        case 0: LAUNCH_S(0); break;
+        case 1: LAUNCH_S(1); break;
+        case 2: LAUNCH_S(2); break;
+        case 3: LAUNCH_S(3); break;
+        case 4: LAUNCH_S(4); break;
+        case 5: LAUNCH_S(5); break;
+        default: LAUNCH_S(6); break;
+        }
+#undef LAUNCH_S
+        HIPCHK(hipGetLastError());
+    }
+    int nb = nblk((long)c->nv * SPMV_G);
+    if (c->precision == 1)
+        k_bsr_spmv<float><<<nb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->valf.p, p, q, c->dmask.p, part, state, it);
+    else
+        k_bsr_spmv<double><<<nb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->val.p, p, q, c->dmask.p, part, state, it);
+    HIPCHK(hipGetLastError());
+    STCHK(tm_end(c, c->tm_spmv));
+    return FIPC_OK;
+}
+extern "C" int fipc_spmv(fipc_ctx* c, const double* x, double* y)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    size_t n = (size_t)c->nv * 24;
+    HIPCHK(hipMemcpyAsync(c->pp.p, x, n, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemsetAsync(c->q.p, 0, n, c->s));
+    STCHK(spmv(c, c->pp.p, c->q.p, nullptr, nullptr, 0));
+    HIPCHK(hipMemcpyAsync(y, c->q.p, n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return FIPC_OK;
+}
+
+extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double* relres)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    int nv = c->nv;
+    int nbv = nblk(nv);
+    int nbs = nblk((long)nv * SPMV_G);
+    // x0 = prescribed Dirichlet increments; q = A x0
+    k_pcg_x0<<<nbv, TPB, 0, c->s>>>(nv, c->xt.p, c->x.p, c->dmask.p, c->sol.p, c->q.p);
+    HIPCHK(hipGetLastError());
+    STCHK(spmv(c, c->sol.p, c->q.p, nullptr, nullptr, 0));
+    k_pcg_init<<<nbv, TPB, 0, c->s>>>(nv, c->rhs.p, c->q.p, c->dmask.p, c->Minv.p, c->r.p, c->z.p, c->pp.p, c->q.p,
+                                       c->part_a.p, c->part_b.p);
+    HIPCHK(hipGetLastError());
+    k_pcg_scalars<<<1, TPB, 0, c->s>>>(c->part_a.p, c->part_b.p, nbv, c->scal.p, c->pcg_state.p);
+    HIPCHK(hipGetLastError());
+    int it = 0, state[2] = {0, 0};
+    const int chunk = 16;
+    double tol2 = tol * tol;
+    while (it < maxit) {
+        int end = std::min(maxit, it + chunk);
+        for (; it < end; it++) {
+            STCHK(spmv(c, c->pp.p, c->q.p, c->part_c.p, c->pcg_state.p, it));
+            k_pcg_upd1<<<nbv, TPB, 0, c->s>>>(nv, c->part_c.p, nbs, c->scal.p, c->dmask.p, c->Minv.p, c->pp.p, c->q.p,
+                                               c->sol.p, c->r.p, c->z.p, c->part_a.p, c->part_b.p, c->pcg_state.p, it);
+            k_pcg_upd2<<<nblk(nv * 3), TPB, 0, c->s>>>(nv, c->part_a.p, c->part_b.p, nbv, c->scal.p, c->z.p, c->pp.p,
+                                                        c->q.p, c->pcg_state.p, it, tol2);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        if (state[0] != 0x7fffffff) break;
+    }
+    double sc[5];
+    HIPCHK(hipMemcpyAsync(sc, c->scal.p, 40, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    int n_it = state[0] == -1 ? 0 : state[1];
+    double rel = sc[2] > 0 ? sqrt(sc[4] / sc[2]) : 0.0;
+    if (state[0] == -1) rel = 0.0;
+    c->last_pcg_iters = n_it;
+    c->last_pcg_relres = rel;
+    if (iters) *iters = n_it;
+    if (relres) *relres = rel;
+    if (!(rel == rel)) return FIPC_ERR_NONFINITE;
+    if (state[0] == 0x7fffffff) return FIPC_ERR_NOT_CONVERGED;
+    return FIPC_OK;
+}
+
+// ---- energy ----
+extern "C" int fipc_energy(fipc_ctx* c, double* E, double* parts)
+{
+    if (!c || !c->nv || !E) return FIPC_ERR_STATE;
+    STCHK(tm_begin(c, c->tm_energy));
+    int nbv = nblk(c->nv), nbt = nblk(std::max(c->nt, 1));
+    double* ep = c->epart.p;
+    k_energy_inertia<<<nbv, TPB, 0, c->s>>>(c->nv, c->m.p, c->x.p, c->xt.p, ep);
+    if (c->nt) k_energy_elastic<<<nbt, TPB, 0, c->s>>>(c->nt, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p, c->x.p, c->dt * c->dt, ep + nbv);
+    HIPCHK(hipGetLastError());
+    std::vector<int> nbc(NTYPES, 0);
+    size_t off = nbv + (c->nt ? nbt : 0);
+    size_t total = off;
+    for (int t = 0; t < NTYPES; t++) { nbc[t] = c->count[t] ? nblk(c->count[t]) : 0; total += nbc[t]; }
+    STCHK(ensure(c->epart, total + 8));
+    ep = c->epart.p;
+    if (c->nt == 0) nbt = 0;
+    size_t o = off;
+    for (int t = 0; t < NTYPES; t++) {
+        if (!nbc[t]) continue;
+#define LAUNCH_E(K) k_energy_contact<K><<<nbc[t], TPB, 0, c->s>>>(c->count[t], c->rows[t].p, c->x.p, c->x0.p, c->dHat2, c->kappa, ep + o)
+        switch (t) {
+        case 0: LAUNCH_E(0); break;
+        case 1: LAUNCH_E(1); break;
+        case 2: LAUNCH_E(2); break;
+        case 3: LAUNCH_E(3); break;
+        case 4: LAUNCH_E(4); break;
+        case 5: LAUNCH_E(5); break;
+        default: LAUNCH_E(6); break;
+        }
+#undef LAUNCH_E
+        o += nbc[t];
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(total);
+    HIPCHK(hipMemcpyAsync(h.data(), ep, total * 8, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    STCHK(tm_end(c, c->tm_energy));
+    double ei = 0, ee = 0, eb = 0;
+    for (int i = 0; i < nbv; i++) ei += h[i];
+    for (int i = 0; i < nbt; i++) ee += h[nbv + i];
+    for (size_t i = off; i < total; i++) eb += h[i];
+    *E = ei + ee + eb;
+    if (parts) { parts[0] = ei; parts[1] = ee; parts[2] = eb; }
+    if (!(*E == *E)) return FIPC_ERR_NONFINITE;
+    return FIPC_OK;
+}
+
+extern "C" int fipc_min_distance(fipc_ctx* c, double* d2min)
+{
+    if (!c || !c->nv || !d2min) return FIPC_ERR_STATE;
+    double big = 1999999999.0;
+    unsigned long long init;
+    memcpy(&init, &big, 8);
+    HIPCHK(hipMemcpyAsync(c->amin.p, &init, 8, hipMemcpyHostToDevice, c->s));
+    for (int t = 0; t < NTYPES; t++) {
+        int n = c->count[t];
+        if (!n) continue;
+#define LAUNCH_M(K) k_min_dist<K><<<nblk(n), TPB, 0, c->s>>>(n, c->rows[t].p, c->x.p, c->amin.p)
+        switch (t) {
+        case 0: LAUNCH_M(0); break;
+        case 1: LAUNCH_M(1); break;
+        case 2: LAUNCH_M(2); break;
+        case 3: LAUNCH_M(3); break;
+        case 4: LAUNCH_M(4); break;
+        case 5: LAUNCH_M(5); break;
+        default: LAUNCH_M(6); break;
+        }
+#undef LAUNCH_M
+    }
+    HIPCHK(hipGetLastError());
+    unsigned long long r;
+    HIPCHK(hipMemcpyAsync(&r, c->amin.p, 8, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    double d;
+    memcpy(&d, &r, 8);
+    *d2min = d;
+    return FIPC_OK;
+}
+
+int fipc_inversion_alpha(fipc_ctx* c, unsigned long long* dst)
+{
+    if (!c->nt) return FIPC_OK;
+    k_inversion<<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->tets.p, c->x.p, c->sol.p, dst);
+    HIPCHK(hipGetLastError());
+    return FIPC_OK;
+}
+
+extern "C" int fipc_get_bsr(fipc_ctx* c, int* nnzb, int* rowptr, int* col, double* val)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    if (nnzb) *nnzb = c->nnzb;
+    if (rowptr) HIPCHK(hipMemcpyAsync(rowptr, c->rowptr.p, (c->nv + 1) * 4, hipMemcpyDeviceToHost, c->s));
+    if (col) HIPCHK(hipMemcpyAsync(col, c->col.p, (size_t)c->nnzb * 4, hipMemcpyDeviceToHost, c->s));
+    if (val) {
+        std::vector<double> planes((size_t)c->nnzb * 9);
+        HIPCHK(hipMemcpyAsync(planes.data(), c->val.p, planes.size() * 8, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        for (size_t b = 0; b < (size_t)c->nnzb; b++)
+            for (int k = 0; k < 9; k++) val[b * 9 + k] = planes[(size_t)k * c->nnzb + b];
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    return FIPC_OK;
+}
+
+extern "C" int fipc_enable_timing(fipc_ctx* c, int on)
+{
+    if (!c) return FIPC_ERR_ARG;
+    c->timing = on != 0;
+    KernelTimer* ts[] = {&c->tm_spmv, &c->tm_assemble, &c->tm_ccd, &c->tm_find, &c->tm_energy, &c->tm_contact};
+    for (auto* t : ts) { t->total_ms = 0; t->launches = 0; }
+    return FIPC_OK;
+}
+extern "C" int fipc_kernel_times(fipc_ctx* c, double* ms, long* launches)
+{
+    if (!c) return FIPC_ERR_ARG;
+    KernelTimer* ts[] = {&c->tm_spmv, &c->tm_assemble, &c->tm_ccd, &c->tm_find, &c->tm_energy, &c->tm_contact};
+    for (int i = 0; i < 6; i++) {
+        if (ms) ms[i] = ts[i]->total_ms;
+        if (launches) launches[i] = ts[i]->launches;
+    }
+    return FIPC_OK;
+}
+extern "C" int fipc_sync(fipc_ctx* c)
+{
+    if (!c) return FIPC_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->s));
+    return FIPC_OK;
+}
+
+// ---- batched per-constraint / per-tet evaluation (tests, drivers) ----
+extern "C" int fipc_batch_constraint(int type, int n, const double* x, const double* x0, double dHat2, double kappa,
+                                     int project, double* E, double* g, double* H)
+{
+    if (type < 0 || type >= NTYPES || n < 0) return FIPC_ERR_ARG;
+    if (n == 0) return FIPC_OK;
+    int K = type == 0 ? 2 : (type == 1 ? 3 : 4);
+    int W = K, N = 3 * K, G = K == 2 ? 2 : 4;
+    // the kernel reads rows of vertex ids: make identity rows over the packed points
+    std::vector<int> rows((size_t)n * W);
+    for (int r = 0; r < n; r++)
+        for (int a = 0; a < W; a++) rows[(size_t)r * W + a] = r * K + a;
+    double *dx, *dx0, *dE, *dg, *dH;
+    int* drows;
+    size_t npts = (size_t)n * K * 3;
+    HIPCHK(hipMalloc(&dx, npts * 8));
+    HIPCHK(hipMalloc(&dx0, npts * 8));
+    HIPCHK(hipMalloc(&dE, (size_t)n * 8));
+    HIPCHK(hipMalloc(&dg, (size_t)n * N * 8));
+    HIPCHK(hipMalloc(&dH, (size_t)n * N * N * 8));
+    HIPCHK(hipMalloc(&drows, rows.size() * 4));
+    HIPCHK(hipMemcpy(dx, x, npts * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dx0, x0 ? x0 : x, npts * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(drows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
+    int nb = nblk((long)n * G);
+#define LAUNCH_B(KK) k_contact<KK><<<nb, TPB>>>(n, drows, dx, dx0, dHat2, kappa, dH, nullptr, nullptr, dE, dg, project, 0)
+    switch (type) {
+    case 0: LAUNCH_B(0); break;
+    case 1: LAUNCH_B(1); break;
+    case 2: LAUNCH_B(2); break;
+    case 3: LAUNCH_B(3); break;
+    case 4: LAUNCH_B(4); break;
+    case 5: LAUNCH_B(5); break;
+    default: LAUNCH_B(6); break;
+    }
+#undef LAUNCH_B
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    if (E) HIPCHK(hipMemcpy(E, dE, (size_t)n * 8, hipMemcpyDeviceToHost));
+    if (g) HIPCHK(hipMemcpy(g, dg, (size_t)n * N * 8, hipMemcpyDeviceToHost));
+    if (H) HIPCHK(hipMemcpy(H, dH, (size_t)n * N * N * 8, hipMemcpyDeviceToHost));
+    (void)hipFree(dx); (void)hipFree(dx0); (void)hipFree(dE); (void)hipFree(dg); (void)hipFree(dH); (void)hipFree(drows);
+    return FIPC_OK;
+}
+
+extern "C" int fipc_batch_tet(int n, const double* F, const double* IB, const double* scale, const double* la,
+                              const double* mu, double* psi, double* P, double* Hout)
+{
+    if (n < 0 || !F || !IB || !scale || !la || !mu || !psi || !P) return FIPC_ERR_ARG;
+    if (n == 0) return FIPC_OK;
+    double *dF, *dIB, *ds, *dla, *dmu, *dpsi, *dP, *dH = nullptr;
+    HIPCHK(hipMalloc(&dF, (size_t)n * 72)); HIPCHK(hipMalloc(&dIB, (size_t)n * 72)); HIPCHK(hipMalloc(&ds, (size_t)n * 8));
+    HIPCHK(hipMalloc(&dla, (size_t)n * 8)); HIPCHK(hipMalloc(&dmu, (size_t)n * 8)); HIPCHK(hipMalloc(&dpsi, (size_t)n * 8));
+    HIPCHK(hipMalloc(&dP, (size_t)n * 72));
+    if (Hout) HIPCHK(hipMalloc(&dH, (size_t)n * 144 * 8));
+    HIPCHK(hipMemcpy(dF, F, (size_t)n * 72, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dIB, IB, (size_t)n * 72, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ds, scale, (size_t)n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dla, la, (size_t)n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dmu, mu, (size_t)n * 8, hipMemcpyHostToDevice));
+    k_batch_tet<<<nblk(n), TPB>>>(n, dF, dIB, ds, dla, dmu, dpsi, dP, dH);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(psi, dpsi, (size_t)n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(P, dP, (size_t)n * 72, hipMemcpyDeviceToHost));
+    if (Hout) HIPCHK(hipMemcpy(Hout, dH, (size_t)n * 144 * 8, hipMemcpyDeviceToHost));
+    (void)hipFree(dF); (void)hipFree(dIB); (void)hipFree(ds); (void)hipFree(dla); (void)hipFree(dmu); (void)hipFree(dpsi);
+    (void)hipFree(dP);
+    if (dH) (void)hipFree(dH);
+    return FIPC_OK;
+}

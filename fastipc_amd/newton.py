@@ -1,0 +1,279 @@
+"""Projected-Newton IPC time stepping on one MI355X.
+
+Host driver of ``projects/FastIPC/NEWTON_IPC.py``: the module-level step
+functions of the reference become methods with the same names, each calling
+one batched entry point of libfastipc.so, and :meth:`NewtonIPC.step` runs the
+reference's frame loop (NEWTON_IPC.py:955-1002):
+
+* Newton iterations stop when ``max|p| < tol`` *before* the line search
+  (:975), the final direction is not applied;
+* the line search takes the CCD / inversion-free step (:980-982), halves while
+  an edge crosses a triangle (:983-985), halves while ``E > E0`` re-running
+  the constraint search (:986-992), then halves on collision once more
+  (:993-995).
+
+Differences, all documented: the linear solve is a block-Jacobi PCG instead
+of CHOLMOD; Dirichlet vertices whose prescribed position moves between steps
+reach it through a prescribed increment in the solve (static Dirichlet
+vertices behave exactly as in the reference); safety caps bound the Newton and
+halving loops, which the reference leaves unbounded.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .model import SceneSetup
+from .timer import Timers
+
+TYPE_NAMES = ("PP", "PE", "PT", "EE", "EEM", "PPM", "PEM")
+TYPE_WIDTH = (2, 3, 4, 4, 4, 4, 4)
+
+
+@dataclass
+class StepStats:
+    newton_iters: int = 0
+    pcg_iters: list = field(default_factory=list)
+    alphas: list = field(default_factory=list)
+    residuals: list = field(default_factory=list)
+    counts: list = field(default_factory=list)
+    energy: float = float("nan")
+    converged: bool = True
+
+
+class NewtonIPC:
+    def __init__(self, scene, precision: int = 0, pcg_tol: float = 1e-6, pcg_maxit: int = 20000,
+                 max_newton: int = 200, max_halvings: int = 60, inversion: bool = True, timing: bool = False):
+        self.so = _lib.load()
+        self.scene = scene
+        self.setup = s = SceneSetup(scene)
+        self.precision = precision
+        self.pcg_tol = pcg_tol
+        self.pcg_maxit = pcg_maxit
+        self.max_newton = max_newton
+        self.max_halvings = max_halvings
+        self.inversion = inversion
+        self.nv = len(s.x0)
+        self.timers = Timers()
+        ctx = C.c_void_p()
+        _lib.call("fipc_create", C.byref(ctx))
+        self.ctx = ctx
+        _lib.call("fipc_upload_mesh", ctx, self.nv, _lib.dptr(s.x0), _lib.dptr(np.ascontiguousarray(s.mass)),
+                  len(s.tets), _lib.iptr(s.tets), _lib.dptr(np.ascontiguousarray(s.IB.reshape(-1, 9))),
+                  _lib.dptr(np.ascontiguousarray(s.vol)), _lib.dptr(_lib.f64(s.la)), _lib.dptr(_lib.f64(s.mu)),
+                  len(s.bp), _lib.iptr(s.bp), len(s.be), _lib.iptr(s.be), len(s.bt), _lib.iptr(s.bt),
+                  s.dirichlet.ctypes.data_as(C.POINTER(C.c_ubyte)))
+        _lib.call("fipc_set_params", ctx, s.dt, s.dHat, s.kappa, s.inv_dx, s.gravity)
+        if timing:
+            _lib.call("fipc_enable_timing", ctx, 1)
+        self.frame = 0
+        self.counts = [0] * 7
+
+    def close(self):
+        if self.ctx:
+            _lib.call("fipc_destroy", self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ state
+    @property
+    def x(self) -> np.ndarray:
+        out = np.empty((self.nv, 3))
+        _lib.call("fipc_get_state", self.ctx, _lib.dptr(out), None)
+        return out
+
+    @property
+    def v(self) -> np.ndarray:
+        out = np.empty((self.nv, 3))
+        _lib.call("fipc_get_state", self.ctx, None, _lib.dptr(out))
+        return out
+
+    def set_state(self, x=None, v=None):
+        xa = _lib.f64(x) if x is not None else None
+        va = _lib.f64(v) if v is not None else None
+        _lib.call("fipc_set_state", self.ctx, _lib.dptr(xa) if xa is not None else None,
+                  _lib.dptr(va) if va is not None else None)
+
+    def vector(self, which: int) -> np.ndarray:
+        out = np.empty((self.nv, 3))
+        _lib.call("fipc_get_vector", self.ctx, which, _lib.dptr(out))
+        return out
+
+    def set_vector(self, which: int, a):
+        a = _lib.f64(a).reshape(self.nv, 3)
+        _lib.call("fipc_set_vector", self.ctx, which, _lib.dptr(a))
+
+    # ------------------------------------------------- reference step functions
+    def compute_xn_and_xTilde(self, t: float):
+        """compute_xn_and_xTilde + move_nodes (NEWTON_IPC.py:497-511)."""
+        target = _lib.f64(self.scene.dirichlet(t))
+        _lib.call("fipc_begin_step", self.ctx, _lib.dptr(target))
+
+    def find_constraints(self):
+        counts = np.zeros(7, np.int32)
+        _lib.call("fipc_find_constraints", self.ctx, _lib.iptr(counts))
+        self.counts = [int(c) for c in counts]
+        return self.counts
+
+    def constraints(self, sort: bool = True):
+        """Host copy of the seven row sets, in np.unique order when ``sort``."""
+        out = {}
+        for t, name in enumerate(TYPE_NAMES):
+            n = self.counts[t]
+            rows = np.zeros((n, TYPE_WIDTH[t]), np.int32)
+            if n:
+                _lib.call("fipc_get_constraints", self.ctx, t, _lib.iptr(rows))
+            if sort and n:
+                rows = np.unique(rows, axis=0)
+            out[name] = rows
+        return out
+
+    def output_current_minimal_distance(self) -> float:
+        d = C.c_double()
+        _lib.call("fipc_min_distance", self.ctx, C.byref(d))
+        return d.value
+
+    def compute_hessian_and_gradient(self):
+        _lib.call("fipc_assemble", self.ctx, self.precision)
+
+    def solve_system(self, t: float = 0.0):
+        it = C.c_int()
+        rel = C.c_double()
+        code = self.so.fipc_solve(self.ctx, self.pcg_tol, self.pcg_maxit, C.byref(it), C.byref(rel))
+        if code not in (0, 5):
+            raise _lib.FipcError("fipc_solve", code)
+        return it.value, rel.value
+
+    def output_residual(self) -> float:
+        r = C.c_double()
+        _lib.call("fipc_residual", self.ctx, C.byref(r))
+        return r.value
+
+    def compute_energy(self, parts: bool = False):
+        e = C.c_double()
+        p = np.zeros(3)
+        _lib.call("fipc_energy", self.ctx, C.byref(e), _lib.dptr(p))
+        return (e.value, p) if parts else e.value
+
+    def compute_intersection_free_step_size(self, parts: bool = False):
+        a = C.c_double()
+        p = np.zeros(3)
+        _lib.call("fipc_ccd_alpha", self.ctx, self.setup.eta, int(self.inversion), C.byref(a), _lib.dptr(p))
+        return (a.value, p) if parts else a.value
+
+    def check_collision(self) -> int:
+        h = C.c_int()
+        _lib.call("fipc_check_collision", self.ctx, C.byref(h))
+        return h.value
+
+    def save_xPrev(self):
+        _lib.call("fipc_save_prev", self.ctx)
+
+    def apply_sol(self, alpha: float):
+        _lib.call("fipc_apply_sol", self.ctx, float(alpha))
+
+    def compute_v(self):
+        _lib.call("fipc_end_step", self.ctx)
+
+    # --------------------------------------------------------------- frame loop
+    def step(self, warm_start=None) -> StepStats:
+        """One time step (NEWTON_IPC.py:956-999).  ``warm_start`` optionally
+        returns a displacement increment to try before the first Newton
+        iteration; it is kept only if it is intersection-free and lowers the
+        incremental potential, otherwise x = x_n is used as in the reference."""
+        st = StepStats()
+        T = self.timers
+        f = self.frame
+        dt = self.setup.dt
+        with T("step"):
+            self.compute_xn_and_xTilde(f * dt)
+            if warm_start is not None:
+                with T("step/warm_start"):
+                    self._try_warm_start(warm_start)
+            while True:
+                st.newton_iters += 1
+                with T("step/build"):
+                    with T("step/build/find"):
+                        st.counts.append(self.find_constraints())
+                    with T("step/build/assemble"):
+                        self.compute_hessian_and_gradient()
+                with T("step/solve"):
+                    it, rel = self.solve_system(f * dt)
+                    st.pcg_iters.append(it)
+                res = self.output_residual()
+                st.residuals.append(res)
+                if res < self.setup.newton_tol:
+                    break
+                if st.newton_iters >= self.max_newton:
+                    st.converged = False
+                    break
+                with T("step/line_search"):
+                    alpha = self._line_search()
+                    st.alphas.append(alpha)
+            self.compute_v()
+        st.energy = float("nan")
+        self.frame += 1
+        T.end_round()
+        return st
+
+    def _line_search(self) -> float:
+        T = self.timers
+        with T("step/line_search/energy"):
+            E0 = self.compute_energy()
+        self.save_xPrev()
+        with T("step/line_search/ccd"):
+            alpha = self.compute_intersection_free_step_size()
+        self.apply_sol(alpha)
+        n = 0
+        with T("step/line_search/collision"):
+            while self.check_collision() and n < self.max_halvings:
+                alpha /= 2.0
+                self.apply_sol(alpha)
+                n += 1
+        with T("step/line_search/find"):
+            self.find_constraints()
+        with T("step/line_search/energy"):
+            E = self.compute_energy()
+        n = 0
+        while E > E0 and n < self.max_halvings:
+            alpha *= 0.5
+            self.apply_sol(alpha)
+            with T("step/line_search/find"):
+                self.find_constraints()
+            with T("step/line_search/energy"):
+                E = self.compute_energy()
+            n += 1
+        n = 0
+        with T("step/line_search/collision"):
+            while self.check_collision() and n < self.max_halvings:
+                alpha /= 2.0
+                self.apply_sol(alpha)
+                n += 1
+        return alpha
+
+    def _try_warm_start(self, fn):
+        dx = fn(self)
+        if dx is None:
+            return
+        x_n = self.x
+        self.find_constraints()
+        E0 = self.compute_energy()
+        self.save_xPrev()
+        self.set_vector(1, dx)
+        alpha = min(1.0, self.compute_intersection_free_step_size())
+        self.apply_sol(alpha)
+        if self.check_collision():
+            self.set_state(x=x_n)
+            return
+        self.find_constraints()
+        if self.compute_energy() > E0:
+            self.set_state(x=x_n)

@@ -1,0 +1,168 @@
+/*
+ * fastipc.h -- C ABI of libfastipc.so, the MI355X (gfx950) implementation of
+ * the FastIPC projected-Newton IPC time step.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Scalar drop-ins with the names, argument order and by-value / double*
+ *     convention of the reference's common/math/wrapper/wrapper.cpp, so that
+ *     every ti.external_func_call site of the reference can bind them
+ *     unchanged (FP64, reentrant, host-side).
+ *
+ *  2. Batched device entry points on a context (fipc_*), one per step
+ *     function of projects/FastIPC/NEWTON_IPC.py.  All return 0 on success or
+ *     a FIPC_ERR_* status.  A context owns its device buffers; calls on one
+ *     context must come from one host thread and are ordered on the
+ *     context's HIP stream.  Host arrays are copied in and out.
+ *
+ * Vertex arrays are float64 (nv, 3) row-major; index arrays are int32.
+ * Constraint types are numbered as the reference's row sets:
+ *   0 PP (2 ints/row), 1 PE (3), 2 PT (4), 3 EE (4), 4 EEM (4), 5 PPM (4), 6 PEM (4)
+ */
+#ifndef FASTIPC_H
+#define FASTIPC_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FIPC_OK 0
+#define FIPC_ERR_ARG 1          /* invalid argument */
+#define FIPC_ERR_HIP 2          /* HIP runtime error */
+#define FIPC_ERR_CAPACITY 3     /* index range exceeds the packed-key limit (2^21 vertices) */
+#define FIPC_ERR_NONFINITE 4    /* NaN/Inf in a result */
+#define FIPC_ERR_NOT_CONVERGED 5 /* PCG hit its iteration limit */
+#define FIPC_ERR_STATE 6        /* call out of order (e.g. no mesh) */
+
+#define FIPC_TYPES 7
+
+/* ---------------------------------------------------------------------- */
+/* 1. scalar drop-ins                                                       */
+/* ---------------------------------------------------------------------- */
+/* replaces wrapper.cpp:50-56 project_pd_3 (JGSL::makePD, UTILS.h:7-25) */
+void project_pd_3(double in_0, double in_1, double in_2, double in_3, double in_4, double in_5,
+                  double in_6, double in_7, double in_8,
+                  double* out_0, double* out_1, double* out_2, double* out_3, double* out_4,
+                  double* out_5, double* out_6, double* out_7, double* out_8);
+/* replaces wrapper.cpp:66-88 project_pd_6 / project_pd_9 / project_pd_12: n*n
+ * doubles by value (row-major) followed by n*n double* outputs */
+void project_pd_6(double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double,
+                   double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*);
+void project_pd_9(double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double,
+                   double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*);
+void project_pd_12(double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double,
+                   double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*);
+/* array forms of the same projection, for callers that are not Taichi */
+void fipc_project_pd(int n, const double* in, double* out);
+/* replaces wrapper.cpp:27-40 svd_3 (rotation-variant SVD, det U = det V = 1) */
+void svd_3(double F00, double F01, double F02, double F10, double F11, double F12, double F20, double F21, double F22,
+           double* U00, double* U01, double* U02, double* U10, double* U11, double* U12, double* U20, double* U21, double* U22,
+           double* s00, double* s01, double* s02, double* s10, double* s11, double* s12, double* s20, double* s21, double* s22,
+           double* V00, double* V01, double* V02, double* V10, double* V11, double* V12, double* V20, double* V21, double* V22);
+/* replaces wrapper.cpp:200-218 */
+void get_smallest_positive_real_quad_root(double a, double b, double c, double tol, double* ret);
+/* replaces wrapper.cpp:220-252 */
+void get_smallest_positive_real_cubic_root(double a, double b, double c, double d, double tol, double* ret);
+/* replaces wrapper.cpp:254-290 (CTCD::vertexFaceCTCD, EVCTCD/CTCD.cpp:383-472) */
+void point_triangle_ccd(double p0, double p1, double p2, double t00, double t01, double t02,
+                        double t10, double t11, double t12, double t20, double t21, double t22,
+                        double dp0, double dp1, double dp2, double dt00, double dt01, double dt02,
+                        double dt10, double dt11, double dt12, double dt20, double dt21, double dt22,
+                        double eta, double dist2_cur, double* ret);
+/* replaces wrapper.cpp:292-328 (CTCD::edgeEdgeCTCD, EVCTCD/CTCD.cpp:245-381) */
+void edge_edge_ccd(double a00, double a01, double a02, double a10, double a11, double a12,
+                   double b00, double b01, double b02, double b10, double b11, double b12,
+                   double da00, double da01, double da02, double da10, double da11, double da12,
+                   double db00, double db01, double db02, double db10, double db11, double db12,
+                   double eta, double dist2_cur, double* ret);
+
+/* ---------------------------------------------------------------------- */
+/* 2a. batched per-constraint / per-element functions on the device         */
+/* ---------------------------------------------------------------------- */
+/* {PP,PE,PT,EE,EEM,PPM,PEM}_g_and_H (common/math/ipc.py:66-329) for n
+ * constraints of one type.  x: n*k*3 points (k = 2,3,4 by type; mollified
+ * types also read x0: n*4*3 rest points).  Outputs E[n], g[n*3k],
+ * H[n*3k*3k] (row-major, SPD-projected when project != 0).  Runs the same
+ * kernels as the assembly. */
+int fipc_batch_constraint(int type, int n, const double* x, const double* x0, double dHat2, double kappa,
+                          int project, double* E, double* g, double* H);
+/* neo-Hookean tet (common/physics/neo_hookean.py + NEWTON_IPC.py:568-691):
+ * F[n*9] row-major -> Psi[n], P[n*9], and the 12x12 Hessian of the tet
+ * energy with respect to its vertices for a rest shape with D_m^{-1} = IB[n*9],
+ * scaled by scale[n] (dt^2 vol0).  Hout may be NULL. */
+int fipc_batch_tet(int n, const double* F, const double* IB, const double* scale, const double* la,
+                   const double* mu, double* psi, double* P, double* Hout);
+/* CCD on the device: type 0 point-triangle, 1 edge-edge.  pts[n*12], disp[n*12],
+ * dist2[n] -> toc[n] (wrapper.cpp:254-328 semantics) */
+int fipc_batch_ccd(int type, int n, const double* pts, const double* disp, double eta, const double* dist2, double* toc);
+
+/* ---------------------------------------------------------------------- */
+/* 2b. the Newton step on a context                                        */
+/* ---------------------------------------------------------------------- */
+typedef struct fipc_ctx fipc_ctx;
+
+int fipc_create(fipc_ctx** out);
+int fipc_destroy(fipc_ctx* c);
+/* mesh and rest state.  IB = D_m^{-1} per tet (row-major 3x3), vol = rest
+ * volume, la/mu Lame parameters, m lumped vertex mass, x0 rest positions
+ * (used by the mollifier), boundary points/edges/triangles
+ * (common/math/graph_tools.py:26-45), dirichlet: nv bytes, nonzero = fixed
+ * (NEWTON_IPC.py:505-511, :750-772). */
+int fipc_upload_mesh(fipc_ctx* c, int nv, const double* x0, const double* m, int nt, const int* tets,
+                     const double* IB, const double* vol, const double* la, const double* mu,
+                     int nbp, const int* bp, int nbe, const int* be, int nbt, const int* bt,
+                     const unsigned char* dirichlet);
+/* dt, dHat (not squared), kappa, 1/cell size of the spatial hash, gravity on axis 1 */
+int fipc_set_params(fipc_ctx* c, double dt, double dHat, double kappa, double inv_dx, double gravity);
+int fipc_set_state(fipc_ctx* c, const double* x, const double* v);
+int fipc_get_state(fipc_ctx* c, double* x, double* v);
+/* compute_xn_and_xTilde + move_nodes (NEWTON_IPC.py:497-511): xn = x,
+ * xTilde = x + dt v + dt^2 g e_1; Dirichlet vertices get xTilde = target
+ * (target: nv*3, only Dirichlet rows are read).  x of Dirichlet vertices is
+ * left where it is; the solve prescribes their increment xTilde - x. */
+int fipc_begin_step(fipc_ctx* c, const double* target);
+/* compute_v (NEWTON_IPC.py:799-802) */
+int fipc_end_step(fipc_ctx* c);
+/* find_constraints (NEWTON_IPC.py:337-357): counts[7] after deduplication */
+int fipc_find_constraints(fipc_ctx* c, int* counts);
+/* rows of one type in lexicographic order for PP/PE (np.unique order) and in
+ * device order for the others (already unique); rows_out: counts[type]*width */
+int fipc_get_constraints(fipc_ctx* c, int type, int* rows_out);
+/* output_current_minimal_distance (NEWTON_IPC.py:815-839) */
+int fipc_min_distance(fipc_ctx* c, double* d2min);
+/* compute_hessian_and_gradient (NEWTON_IPC.py:736-747): precision 0 = FP64,
+ * 1 = FP32 Hessian storage (contact rows closer than 1e-2 dHat stay FP64) */
+int fipc_assemble(fipc_ctx* c, int precision);
+/* solve_system (NEWTON_IPC.py:750-777) with block-Jacobi PCG instead of
+ * CHOLMOD: relative residual tolerance tol, at most maxit iterations */
+int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double* relres);
+/* output_residual (NEWTON_IPC.py:805-812): max |p_i| */
+int fipc_residual(fipc_ctx* c, double* res);
+/* compute_energy (NEWTON_IPC.py:514-542); parts[3] = inertia, elastic, barrier (may be NULL) */
+int fipc_energy(fipc_ctx* c, double* E, double* parts);
+/* compute_intersection_free_step_size (NEWTON_IPC.py:456-468): PT and EE CCD
+ * filters and (inversion != 0) the per-tet inversion filter; parts[3] may be NULL */
+int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* alpha, double* parts);
+/* save_xPrev (NEWTON_IPC.py:786-789) */
+int fipc_save_prev(fipc_ctx* c);
+/* apply_sol (NEWTON_IPC.py:792-796): x = xPrev + alpha p */
+int fipc_apply_sol(fipc_ctx* c, double alpha);
+/* check_collision (NEWTON_IPC.py:855-886) */
+int fipc_check_collision(fipc_ctx* c, int* hit);
+/* accessors for tests and drivers */
+int fipc_get_vector(fipc_ctx* c, int which, double* out);   /* 0 x, 1 p (solution), 2 rhs, 3 xTilde */
+int fipc_set_vector(fipc_ctx* c, int which, const double* in);
+/* BSR of the elastic + inertia Hessian: rowptr[nv+1], col[nnzb], val[nnzb*9] (block row-major) */
+int fipc_get_bsr(fipc_ctx* c, int* nnzb, int* rowptr, int* col, double* val);
+/* y = A x with the assembled Hessian (elastic BSR + contact rows), Dirichlet rows zeroed */
+int fipc_spmv(fipc_ctx* c, const double* x, double* y);
+/* timing of the hot kernels (HIP events on the context stream):
+ * ms[6] = total ms of SpMV, assembly, CCD, broad phase, energy, contact
+ * launches[6] = number of launches */
+int fipc_enable_timing(fipc_ctx* c, int on);
+int fipc_kernel_times(fipc_ctx* c, double* ms, long* launches);
+int fipc_sync(fipc_ctx* c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
