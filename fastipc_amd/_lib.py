@@ -62,6 +62,8 @@ _SIGS = {
     "fipc_batch_constraint": [C.c_int, C.c_int, _P, _P, _D, _D, C.c_int, _P, _P, _P],
     "fipc_batch_tet": [C.c_int, _P, _P, _P, _P, _P, _P, _P, _P],
     "fipc_batch_ccd": [C.c_int, C.c_int, _P, _P, _D, _P, _P],
+    "fipc_host_constraint": [C.c_int, _P, _P, _D, _D, C.c_int, _P, _P, _P],
+    "fipc_host_tet": [_P, _P, _D, _D, _D, _P, _P, _P],
 }
 _SCALAR = {
     "project_pd_3": [_D] * 9 + [_P] * 9,

@@ -25,6 +25,9 @@ CFLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # hardware FP64/FP32 atomics on coarse-grained device memory
     "-munsafe-fp-atomics",
+    # no a*b+c contraction: classification and CCD decisions must round
+    # exactly like the strict-IEEE oracle
+    "-ffp-contract=off",
     "-Wno-unused-result",
 ]
 

@@ -180,6 +180,7 @@ __global__ void __launch_bounds__(TPB) k_grid_query(int nq, int nchunks, const i
 static int grid_build(fipc_ctx* c, int n, const BoxFn& box)
 {
     Grid& g = c->grid;
+    STCHK(fipc_tm_begin(c, TM_GRID));
     STCHK(ensure(g.cnt, (size_t)n + 2));
     HIPCHK(hipMemsetAsync(c->flag.p, 0, 4, c->s));
     k_grid_count<<<nblk(n + 1), TPB, 0, c->s>>>(n, box, c->inv_dx, g.cnt.p, c->flag.p);
@@ -219,6 +220,7 @@ static int grid_build(fipc_ctx* c, int n, const BoxFn& box)
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, g.bstart.p, g.bstart.p, M + 1, c->s));
     if (ne) k_grid_scatter<<<nblk(ne), TPB, 0, c->s>>>(ne, g.ekey.p, g.eitem.p, ecell, g.bstart.p, g.bfill.p, g.items.p, cells);
     HIPCHK(hipGetLastError());
+    STCHK(fipc_tm_end(c, TM_GRID));
     return FIPC_OK;
 }
 
@@ -244,16 +246,18 @@ static int query_prep(fipc_ctx* c, int nq, const BoxFn& qbox, DVec<int>& qoff, i
 }
 
 template <class Visit>
-static int run_query(fipc_ctx* c, int nq, const BoxFn& qbox, Visit visit)
+static int run_query(fipc_ctx* c, int nq, const BoxFn& qbox, Visit visit, int timer)
 {
     if (nq == 0 || c->grid.nentries == 0) return FIPC_OK;
     int nchunks = 0;
     STCHK(query_prep(c, nq, qbox, c->grid.qoff, &nchunks));
     if (!nchunks) return FIPC_OK;
     Grid& g = c->grid;
+    STCHK(fipc_tm_begin(c, timer));
     k_grid_query<<<nblk(nchunks), TPB, 0, c->s>>>(nq, nchunks, c->grid.qoff.p, qbox, c->inv_dx, g.M, g.bstart.p, g.items.p,
                                                    c->keys_tmp.p + g.nentries, visit);
     HIPCHK(hipGetLastError());
+    STCHK(fipc_tm_end(c, timer));
     return FIPC_OK;
 }
 
@@ -406,13 +410,13 @@ static int find_once(fipc_ctx* c, int* counts)
     STCHK(grid_build(c, c->nbp, pts));
     BoxFn tris{c->x.p, nullptr, c->bt.p, 2, 0.0};
     VisitPT vpt{c->x.p, c->bp.p, c->bt.p, c->dHat, c->dHat2, sink};
-    STCHK(run_query(c, c->nbt, tris, vpt));
+    STCHK(run_query(c, c->nbt, tris, vpt, TM_FIND));
     // EE: edges inserted with their box, queried with box +- dHat
     BoxFn edges{c->x.p, nullptr, c->be.p, 1, 0.0};
     STCHK(grid_build(c, c->nbe, edges));
     BoxFn qedges{c->x.p, nullptr, c->be.p, 1, c->dHat};
     VisitEE vee{c->x.p, c->x0.p, c->be.p, c->dHat, c->dHat2, sink};
-    STCHK(run_query(c, c->nbe, qedges, vee));
+    STCHK(run_query(c, c->nbe, qedges, vee, TM_FIND));
     HIPCHK(hipMemcpyAsync(counts, c->counters.p, NTYPES * 4, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     return FIPC_OK;
@@ -526,32 +530,20 @@ extern "C" int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* al
     unsigned long long init[3];
     for (int i = 0; i < 3; i++) memcpy(&init[i], &one, 8);
     HIPCHK(hipMemcpyAsync(c->amin.p, init, 24, hipMemcpyHostToDevice, c->s));
-    if (c->timing) {
-        if (!c->tm_ccd.a) { HIPCHK(hipEventCreate(&c->tm_ccd.a)); HIPCHK(hipEventCreate(&c->tm_ccd.b)); }
-        HIPCHK(hipEventRecord(c->tm_ccd.a, c->s));
-    }
     BoxFn pts{c->x.p, c->sol.p, c->bp.p, 0, c->dHat};
     STCHK(grid_build(c, c->nbp, pts));
     BoxFn tris{c->x.p, c->sol.p, c->bt.p, 2, 0.0};
     VisitCCDPT vpt{c->x.p, c->sol.p, c->bp.p, c->bt.p, c->dHat, eta, c->amin.p};
-    STCHK(run_query(c, c->nbt, tris, vpt));
+    STCHK(run_query(c, c->nbt, tris, vpt, TM_CCD));
     BoxFn edges{c->x.p, c->sol.p, c->be.p, 1, 0.0};
     STCHK(grid_build(c, c->nbe, edges));
     BoxFn qedges{c->x.p, c->sol.p, c->be.p, 1, c->dHat};
     VisitCCDEE vee{c->x.p, c->sol.p, c->be.p, c->dHat, eta, c->amin.p + 1};
-    STCHK(run_query(c, c->nbe, qedges, vee));
+    STCHK(run_query(c, c->nbe, qedges, vee, TM_CCD));
     if (inversion) STCHK(fipc_inversion_alpha(c, c->amin.p + 2));
     unsigned long long out[3];
     HIPCHK(hipMemcpyAsync(out, c->amin.p, 24, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
-    if (c->timing) {
-        HIPCHK(hipEventRecord(c->tm_ccd.b, c->s));
-        HIPCHK(hipEventSynchronize(c->tm_ccd.b));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->tm_ccd.a, c->tm_ccd.b));
-        c->tm_ccd.total_ms += ms;
-        c->tm_ccd.launches++;
-    }
     double a[3];
     memcpy(a, out, 24);
     if (parts) { parts[0] = a[0]; parts[1] = a[1]; parts[2] = a[2]; }
@@ -580,7 +572,7 @@ extern "C" int fipc_check_collision(fipc_ctx* c, int* hit)
     HIPCHK(hipMemsetAsync(c->flag.p + 1, 0, 4, c->s));
     BoxFn tris{c->x.p, nullptr, c->bt.p, 2, 0.0};
     VisitCollide v{c->x.p, c->be.p, c->bt.p, c->flag.p + 1};
-    STCHK(run_query(c, c->nbt, tris, v));
+    STCHK(run_query(c, c->nbt, tris, v, TM_FIND));
     int h = 0;
     HIPCHK(hipMemcpyAsync(&h, c->flag.p + 1, 4, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));

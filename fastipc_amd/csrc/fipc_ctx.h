@@ -66,8 +66,12 @@ struct Grid {
     int nentries = 0;
 };
 
+// HIP-event timing of single kernels: pairs are recorded on the context
+// stream without synchronising and folded into totals when read
+#define FIPC_NTIMERS 8
+enum { TM_SPMV = 0, TM_ELASTIC, TM_CCD, TM_FIND, TM_ENERGY, TM_CONTACT, TM_PCGVEC, TM_GRID };
 struct KernelTimer {
-    hipEvent_t a = nullptr, b = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     double total_ms = 0.0;
     long launches = 0;
 };
@@ -112,8 +116,11 @@ struct fipc_ctx {
     // statistics
     double last_pcg_relres = 0.0;
     int last_pcg_iters = 0;
-    KernelTimer tm_spmv, tm_assemble, tm_ccd, tm_find, tm_energy, tm_contact;
+    KernelTimer tm[FIPC_NTIMERS];
+    std::vector<hipEvent_t> event_pool;
     bool timing = false;
 };
 
-int fipc_grid_build_points(fipc_ctx* c, int mode, double inflate);
+int fipc_tm_begin(fipc_ctx* c, int which);
+int fipc_tm_end(fipc_ctx* c, int which);
+int fipc_tm_fold(fipc_ctx* c);

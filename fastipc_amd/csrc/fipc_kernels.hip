@@ -136,68 +136,6 @@ __global__ void k_inertia(int nv, int nnzb, const double* __restrict__ m, const 
     for (int k = 0; k < 9; k++) cdiag[9 * i + k] = 0.0;
 }
 
-// 12x12 tet Hessian as Z M Z^T (see fipc_math.h NHFactor), accumulated per
-// 3x3 block.  emit(a, c, blk) receives the block of vertex pair (a, c), c >= a.
-template <class Emit>
-__device__ __forceinline__ void tet_hessian_blocks(const NHFactor& f, const double (&Bm)[4][3], double sc, Emit emit)
-{
-    // bv[a][b] = sum_j Bm[a][j] V[j][b]
-    double bv[4][3];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) bv[a][b] = Bm[a][0] * f.V[0 * 3 + b] + Bm[a][1] * f.V[1 * 3 + b] + Bm[a][2] * f.V[2 * 3 + b];
-    // M in the (ka,kb) pair basis: diagonal pairs via S, off pairs via B blocks
-    // B block k couples idx (i,j) and (j,i) for (i,j) = (0,1), (1,2), (2,0)
-    const int PI[3] = {0, 1, 2}, PJ[3] = {1, 2, 0};
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-        // Y[i][l] for l = ka*3+kb: sum_k z_k[(a,i)] M[k][l]
-        double Y[3][9];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-#pragma unroll
-            for (int l = 0; l < 9; l++) Y[i][l] = 0.0;
-#pragma unroll
-            for (int be = 0; be < 3; be++) {
-                double s = 0.0;
-#pragma unroll
-                for (int al = 0; al < 3; al++) s += bv[a][al] * f.U[i * 3 + al] * f.S[al][be];
-                Y[i][be * 4] = s * sc;
-            }
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const int pi = PI[k], pj = PJ[k];
-                // z_(pi,pj)[(a,i)] = bv[a][pj] U[i][pi]; z_(pj,pi)[(a,i)] = bv[a][pi] U[i][pj]
-                double zij = bv[a][pj] * f.U[i * 3 + pi], zji = bv[a][pi] * f.U[i * 3 + pj];
-                // B block in row-major [[B00, B01], [B10, B11]] on basis ((pi,pj), (pj,pi)),
-                // except pair (2,0) whose block is stored transposed (neo_hookean.py:120-123)
-                double B00, B01, B10, B11;
-                if (k < 2) { B00 = f.B[k][0]; B01 = f.B[k][1]; B10 = f.B[k][2]; B11 = f.B[k][3]; }
-                else { B00 = f.B[k][3]; B01 = f.B[k][2]; B10 = f.B[k][1]; B11 = f.B[k][0]; }
-                Y[i][pi * 3 + pj] = (zij * B00 + zji * B10) * sc;
-                Y[i][pj * 3 + pi] = (zij * B01 + zji * B11) * sc;
-            }
-        }
-#pragma unroll
-        for (int c = a; c < 4; c++) {
-            double blk[9];
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int l = 0; l < 3; l++) {
-                    double s = 0.0;
-#pragma unroll
-                    for (int ka = 0; ka < 3; ka++)
-#pragma unroll
-                        for (int kb = 0; kb < 3; kb++) s += Y[i][ka * 3 + kb] * bv[c][kb] * f.U[l * 3 + ka];
-                    blk[i * 3 + l] = s;
-                }
-            emit(a, c, blk);
-        }
-    }
-}
-
 __device__ __forceinline__ void tet_load(const int* tets, const double* IB, const double* x, int e, int (&t)[4],
                                          double (&F)[9], double (&Bm)[4][3])
 {
@@ -658,22 +596,47 @@ __global__ void k_batch_tet(int n, const double* Fin, const double* IB, const do
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static int tm_begin(fipc_ctx* c, KernelTimer& t)
+static hipEvent_t pool_get(fipc_ctx* c)
+{
+    if (c->event_pool.empty()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+}
+int fipc_tm_begin(fipc_ctx* c, int which)
 {
     if (!c->timing) return FIPC_OK;
-    if (!t.a) { HIPCHK(hipEventCreate(&t.a)); HIPCHK(hipEventCreate(&t.b)); }
-    HIPCHK(hipEventRecord(t.a, c->s));
+    hipEvent_t a = pool_get(c), b = pool_get(c);
+    if (!a || !b) return FIPC_ERR_HIP;
+    HIPCHK(hipEventRecord(a, c->s));
+    c->tm[which].pending.push_back({a, b});
     return FIPC_OK;
 }
-static int tm_end(fipc_ctx* c, KernelTimer& t)
+int fipc_tm_end(fipc_ctx* c, int which)
 {
     if (!c->timing) return FIPC_OK;
-    HIPCHK(hipEventRecord(t.b, c->s));
-    HIPCHK(hipEventSynchronize(t.b));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
-    t.total_ms += ms;
-    t.launches++;
+    HIPCHK(hipEventRecord(c->tm[which].pending.back().second, c->s));
+    if (c->tm[which].pending.size() > 4096) STCHK(fipc_tm_fold(c));
+    return FIPC_OK;
+}
+int fipc_tm_fold(fipc_ctx* c)
+{
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (auto& t : c->tm) {
+        for (auto& pr : t.pending) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+            t.total_ms += ms;
+            t.launches++;
+            c->event_pool.push_back(pr.first);
+            c->event_pool.push_back(pr.second);
+        }
+        t.pending.clear();
+    }
     return FIPC_OK;
 }
 
@@ -713,8 +676,8 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->epart); release(c->amin); release(c->flag);
     release(c->grid.cnt); release(c->grid.ekey); release(c->grid.eitem); release(c->grid.bstart);
     release(c->grid.bfill); release(c->grid.items);
-    KernelTimer* ts[] = {&c->tm_spmv, &c->tm_assemble, &c->tm_ccd, &c->tm_find, &c->tm_energy, &c->tm_contact};
-    for (auto* t : ts) { if (t->a) { (void)hipEventDestroy(t->a); (void)hipEventDestroy(t->b); } }
+    (void)fipc_tm_fold(c);
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->s);
     delete c;
     return FIPC_OK;
@@ -936,18 +899,18 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
 {
     if (!c || !c->nv) return FIPC_ERR_STATE;
     c->precision = precision;
-    STCHK(tm_begin(c, c->tm_assemble));
     HIPCHK(hipMemsetAsync(c->val.p, 0, (size_t)c->nnzb * 9 * 8, c->s));
     k_inertia<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->nnzb, c->m.p, c->x.p, c->xt.p, c->diag.p, c->val.p, c->rhs.p, c->cdiag.p);
     HIPCHK(hipGetLastError());
+    STCHK(fipc_tm_begin(c, TM_ELASTIC));
     if (c->nt)
         k_elastic<<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->nnzb, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p, c->x.p,
                                                   c->slots.p, c->dt * c->dt, c->val.p, c->rhs.p);
     HIPCHK(hipGetLastError());
-    STCHK(tm_end(c, c->tm_assemble));
-    STCHK(tm_begin(c, c->tm_contact));
+    STCHK(fipc_tm_end(c, TM_ELASTIC));
+    STCHK(fipc_tm_begin(c, TM_CONTACT));
     STCHK(launch_contacts(c, 1));
-    STCHK(tm_end(c, c->tm_contact));
+    STCHK(fipc_tm_end(c, TM_CONTACT));
     k_precond<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->nnzb, c->val.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p);
     HIPCHK(hipGetLastError());
     if (precision == 1) {
@@ -975,7 +938,6 @@ int fipc_convert_f32(fipc_ctx* c)
 // ---- SpMV ----
 static int spmv(fipc_ctx* c, const double* p, double* q, double* part, const int* state, int it)
 {
-    STCHK(tm_begin(c, c->tm_spmv));
     for (int t = 0; t < NTYPES; t++) {
         int n = c->count[t];
         if (!n) continue;
@@ -995,12 +957,13 @@ static int spmv(fipc_ctx* c, const double* p, double* q, double* part, const int
         HIPCHK(hipGetLastError());
     }
     int nb = nblk((long)c->nv * SPMV_G);
+    STCHK(fipc_tm_begin(c, TM_SPMV));
     if (c->precision == 1)
         k_bsr_spmv<float><<<nb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->valf.p, p, q, c->dmask.p, part, state, it);
     else
         k_bsr_spmv<double><<<nb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->val.p, p, q, c->dmask.p, part, state, it);
     HIPCHK(hipGetLastError());
-    STCHK(tm_end(c, c->tm_spmv));
+    STCHK(fipc_tm_end(c, TM_SPMV));
     return FIPC_OK;
 }
 extern "C" int fipc_spmv(fipc_ctx* c, const double* x, double* y)
@@ -1037,10 +1000,12 @@ extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double
         int end = std::min(maxit, it + chunk);
         for (; it < end; it++) {
             STCHK(spmv(c, c->pp.p, c->q.p, c->part_c.p, c->pcg_state.p, it));
+            STCHK(fipc_tm_begin(c, TM_PCGVEC));
             k_pcg_upd1<<<nbv, TPB, 0, c->s>>>(nv, c->part_c.p, nbs, c->scal.p, c->dmask.p, c->Minv.p, c->pp.p, c->q.p,
                                                c->sol.p, c->r.p, c->z.p, c->part_a.p, c->part_b.p, c->pcg_state.p, it);
             k_pcg_upd2<<<nblk(nv * 3), TPB, 0, c->s>>>(nv, c->part_a.p, c->part_b.p, nbv, c->scal.p, c->z.p, c->pp.p,
                                                         c->q.p, c->pcg_state.p, it, tol2);
+            STCHK(fipc_tm_end(c, TM_PCGVEC));
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
@@ -1066,7 +1031,7 @@ extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double
 extern "C" int fipc_energy(fipc_ctx* c, double* E, double* parts)
 {
     if (!c || !c->nv || !E) return FIPC_ERR_STATE;
-    STCHK(tm_begin(c, c->tm_energy));
+    STCHK(fipc_tm_begin(c, TM_ENERGY));
     int nbv = nblk(c->nv), nbt = nblk(std::max(c->nt, 1));
     double* ep = c->epart.p;
     k_energy_inertia<<<nbv, TPB, 0, c->s>>>(c->nv, c->m.p, c->x.p, c->xt.p, ep);
@@ -1096,10 +1061,10 @@ extern "C" int fipc_energy(fipc_ctx* c, double* E, double* parts)
         o += nbc[t];
     }
     HIPCHK(hipGetLastError());
+    STCHK(fipc_tm_end(c, TM_ENERGY));
     std::vector<double> h(total);
     HIPCHK(hipMemcpyAsync(h.data(), ep, total * 8, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
-    STCHK(tm_end(c, c->tm_energy));
     double ei = 0, ee = 0, eb = 0;
     for (int i = 0; i < nbv; i++) ei += h[i];
     for (int i = 0; i < nbt; i++) ee += h[nbv + i];
@@ -1170,18 +1135,18 @@ extern "C" int fipc_get_bsr(fipc_ctx* c, int* nnzb, int* rowptr, int* col, doubl
 extern "C" int fipc_enable_timing(fipc_ctx* c, int on)
 {
     if (!c) return FIPC_ERR_ARG;
+    STCHK(fipc_tm_fold(c));
     c->timing = on != 0;
-    KernelTimer* ts[] = {&c->tm_spmv, &c->tm_assemble, &c->tm_ccd, &c->tm_find, &c->tm_energy, &c->tm_contact};
-    for (auto* t : ts) { t->total_ms = 0; t->launches = 0; }
+    for (auto& t : c->tm) { t.total_ms = 0; t.launches = 0; }
     return FIPC_OK;
 }
 extern "C" int fipc_kernel_times(fipc_ctx* c, double* ms, long* launches)
 {
     if (!c) return FIPC_ERR_ARG;
-    KernelTimer* ts[] = {&c->tm_spmv, &c->tm_assemble, &c->tm_ccd, &c->tm_find, &c->tm_energy, &c->tm_contact};
-    for (int i = 0; i < 6; i++) {
-        if (ms) ms[i] = ts[i]->total_ms;
-        if (launches) launches[i] = ts[i]->launches;
+    STCHK(fipc_tm_fold(c));
+    for (int i = 0; i < FIPC_NTIMERS; i++) {
+        if (ms) ms[i] = c->tm[i].total_ms;
+        if (launches) launches[i] = c->tm[i].launches;
     }
     return FIPC_OK;
 }

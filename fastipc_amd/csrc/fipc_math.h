@@ -609,6 +609,66 @@ FI void nh_factor(const double* F, double la, double mu, NHFactor& o)
     }
 }
 
+// 12x12 tet Hessian as Z M Z^T (see fipc_math.h NHFactor), accumulated per
+// 3x3 block.  emit(a, c, blk) receives the block of vertex pair (a, c), c >= a.
+template <class Emit>
+FI void tet_hessian_blocks(const NHFactor& f, const double (&Bm)[4][3], double sc, Emit emit)
+{
+    // bv[a][b] = sum_j Bm[a][j] V[j][b]
+    double bv[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) bv[a][b] = Bm[a][0] * f.V[0 * 3 + b] + Bm[a][1] * f.V[1 * 3 + b] + Bm[a][2] * f.V[2 * 3 + b];
+    // M in the (ka,kb) pair basis: diagonal pairs via S, off pairs via B blocks
+    // B block k couples idx (i,j) and (j,i) for (i,j) = (0,1), (1,2), (2,0)
+    const int PI[3] = {0, 1, 2}, PJ[3] = {1, 2, 0};
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        // Y[i][l] for l = ka*3+kb: sum_k z_k[(a,i)] M[k][l]
+        double Y[3][9];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+#pragma unroll
+            for (int l = 0; l < 9; l++) Y[i][l] = 0.0;
+#pragma unroll
+            for (int be = 0; be < 3; be++) {
+                double s = 0.0;
+#pragma unroll
+                for (int al = 0; al < 3; al++) s += bv[a][al] * f.U[i * 3 + al] * f.S[al][be];
+                Y[i][be * 4] = s * sc;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int pi = PI[k], pj = PJ[k];
+                // z_(pi,pj)[(a,i)] = bv[a][pj] U[i][pi]; z_(pj,pi)[(a,i)] = bv[a][pi] U[i][pj]
+                double zij = bv[a][pj] * f.U[i * 3 + pi], zji = bv[a][pi] * f.U[i * 3 + pj];
+                // block k on the basis ((pi,pj), (pj,pi)), row-major (neo_hookean.py:112-123;
+                // for k = 2 the basis is (6, 2) in the reference's M indexing)
+                const double B00 = f.B[k][0], B01 = f.B[k][1], B10 = f.B[k][2], B11 = f.B[k][3];
+                Y[i][pi * 3 + pj] = (zij * B00 + zji * B10) * sc;
+                Y[i][pj * 3 + pi] = (zij * B01 + zji * B11) * sc;
+            }
+        }
+#pragma unroll
+        for (int c = a; c < 4; c++) {
+            double blk[9];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int l = 0; l < 3; l++) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int ka = 0; ka < 3; ka++)
+#pragma unroll
+                        for (int kb = 0; kb < 3; kb++) s += Y[i][ka * 3 + kb] * bv[c][kb] * f.U[l * 3 + ka];
+                    blk[i * 3 + l] = s;
+                }
+            emit(a, c, blk);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // roots (common/math/wrapper/wrapper.cpp:200-252) -- Cardano with complex
 // arithmetic; the principal cube root is taken in polar form

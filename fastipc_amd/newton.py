@@ -197,16 +197,16 @@ class NewtonIPC:
         with T("step"):
             self.compute_xn_and_xTilde(f * dt)
             if warm_start is not None:
-                with T("step/warm_start"):
+                with T("warm_start"):
                     self._try_warm_start(warm_start)
             while True:
                 st.newton_iters += 1
-                with T("step/build"):
-                    with T("step/build/find"):
+                with T("build"):
+                    with T("find"):
                         st.counts.append(self.find_constraints())
-                    with T("step/build/assemble"):
+                    with T("assemble"):
                         self.compute_hessian_and_gradient()
-                with T("step/solve"):
+                with T("solve"):
                     it, rel = self.solve_system(f * dt)
                     st.pcg_iters.append(it)
                 res = self.output_residual()
@@ -216,7 +216,7 @@ class NewtonIPC:
                 if st.newton_iters >= self.max_newton:
                     st.converged = False
                     break
-                with T("step/line_search"):
+                with T("line_search"):
                     alpha = self._line_search()
                     st.alphas.append(alpha)
             self.compute_v()
@@ -227,33 +227,33 @@ class NewtonIPC:
 
     def _line_search(self) -> float:
         T = self.timers
-        with T("step/line_search/energy"):
+        with T("energy"):
             E0 = self.compute_energy()
         self.save_xPrev()
-        with T("step/line_search/ccd"):
+        with T("ccd"):
             alpha = self.compute_intersection_free_step_size()
         self.apply_sol(alpha)
         n = 0
-        with T("step/line_search/collision"):
+        with T("collision"):
             while self.check_collision() and n < self.max_halvings:
                 alpha /= 2.0
                 self.apply_sol(alpha)
                 n += 1
-        with T("step/line_search/find"):
+        with T("find"):
             self.find_constraints()
-        with T("step/line_search/energy"):
+        with T("energy"):
             E = self.compute_energy()
         n = 0
         while E > E0 and n < self.max_halvings:
             alpha *= 0.5
             self.apply_sol(alpha)
-            with T("step/line_search/find"):
+            with T("find"):
                 self.find_constraints()
-            with T("step/line_search/energy"):
+            with T("energy"):
                 E = self.compute_energy()
             n += 1
         n = 0
-        with T("step/line_search/collision"):
+        with T("collision"):
             while self.check_collision() and n < self.max_halvings:
                 alpha /= 2.0
                 self.apply_sol(alpha)

@@ -95,6 +95,14 @@ int fipc_batch_tet(int n, const double* F, const double* IB, const double* scale
  * dist2[n] -> toc[n] (wrapper.cpp:254-328 semantics) */
 int fipc_batch_ccd(int type, int n, const double* pts, const double* disp, double eta, const double* dist2, double* toc);
 
+/* host-side evaluation of the same per-constraint / per-tet math that the
+ * kernels run (for CPU tests and debugging; no GPU needed).  Arguments as for
+ * the batched versions with n = 1; fipc_host_tet takes scale = dt^2 vol0. */
+int fipc_host_constraint(int type, const double* x, const double* x0, double dHat2, double kappa, int project,
+                         double* E, double* g, double* H);
+int fipc_host_tet(const double* F, const double* IB, double scale, double la, double mu, double* psi, double* P,
+                  double* H);
+
 /* ---------------------------------------------------------------------- */
 /* 2b. the Newton step on a context                                        */
 /* ---------------------------------------------------------------------- */
@@ -155,9 +163,11 @@ int fipc_set_vector(fipc_ctx* c, int which, const double* in);
 int fipc_get_bsr(fipc_ctx* c, int* nnzb, int* rowptr, int* col, double* val);
 /* y = A x with the assembled Hessian (elastic BSR + contact rows), Dirichlet rows zeroed */
 int fipc_spmv(fipc_ctx* c, const double* x, double* y);
-/* timing of the hot kernels (HIP events on the context stream):
- * ms[6] = total ms of SpMV, assembly, CCD, broad phase, energy, contact
- * launches[6] = number of launches */
+/* timing of the hot kernels with HIP events on the context stream, recorded
+ * without synchronisation while enabled and folded when read:
+ * ms[8] / launches[8] = BSR SpMV, elastic assembly, CCD narrow+broad query,
+ * constraint/collision query, energy, contact assembly, PCG vector updates,
+ * grid build (phase wall time on the stream) */
 int fipc_enable_timing(fipc_ctx* c, int on);
 int fipc_kernel_times(fipc_ctx* c, double* ms, long* launches);
 int fipc_sync(fipc_ctx* c);

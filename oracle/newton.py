@@ -164,7 +164,9 @@ class OracleIPC:
         if _cholesky is not None:
             sol = _cholesky(A)(b)
         else:
-            sol = scipy.sparse.linalg.splu(A).solve(b)
+            # SPD system: symmetric ordering, no threshold pivoting
+            sol = scipy.sparse.linalg.splu(A, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0,
+                                           options=dict(SymmetricMode=True)).solve(b)
         sol[D] = pD[D]
         self.p = sol.reshape(-1, 3)
         self._tick("solve", t0)
