@@ -57,7 +57,7 @@ _SIGS = {
     "fipc_get_bsr": [_VP, _IP, _IP, _IP, _P],
     "fipc_spmv": [_VP, _P, _P],
     "fipc_enable_timing": [_VP, C.c_int],
-    "fipc_kernel_times": [_VP, _P, C.POINTER(C.c_long)],
+    "fipc_kernel_times": [_VP, C.c_int, _P, C.POINTER(C.c_long)],
     "fipc_sync": [_VP],
     "fipc_batch_constraint": [C.c_int, C.c_int, _P, _P, _D, _D, C.c_int, _P, _P, _P],
     "fipc_batch_tet": [C.c_int, _P, _P, _P, _P, _P, _P, _P, _P],

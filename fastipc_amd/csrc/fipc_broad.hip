@@ -40,10 +40,16 @@ __device__ __forceinline__ unsigned long long cell_key(int i, int j, int k)
     const long long B = 1 << 20;
     return ((unsigned long long)(i + B) << 42) | ((unsigned long long)(j + B) << 21) | (unsigned long long)(k + B);
 }
+// bucket of a cell: splitmix64 finaliser of the exact cell key, so that the
+// axis-aligned rows and planes of cells that lattice meshes produce spread
+// evenly over a power-of-two table
 __device__ __forceinline__ unsigned bucket_of(int i, int j, int k, unsigned M)
 {
-    unsigned h = ((unsigned)i * 73856093u) ^ ((unsigned)j * 19349663u) ^ ((unsigned)k * 83492791u);
-    return h & (M - 1);
+    unsigned long long z = cell_key(i, j, k) + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return (unsigned)z & (M - 1);
 }
 
 // ---- boxes ---------------------------------------------------------------
@@ -87,25 +93,30 @@ __global__ void k_grid_count(int n, BoxFn box, double inv, int* cnt, int* overfl
     if (c > (1 << 24) || c < 0) { atomicOr(overflow, 1); c = 0; }
     cnt[i] = (int)c;
 }
-__global__ void k_grid_fill(int n, BoxFn box, double inv, unsigned M, const int* off, unsigned* ekey, int* eitem,
-                            unsigned long long* ecell)
+// one thread per grid entry: the owning item is found by binary search in
+// the scanned counts, so an item covering thousands of cells (a long plate
+// edge) does not serialise a wave
+__global__ void k_grid_fill(int n, int ne, BoxFn box, double inv, unsigned M, const int* __restrict__ off,
+                            unsigned* ekey, int* eitem, unsigned long long* ecell)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    v3 lo, hi;
-    box(i, lo, hi);
-    CellRange r = cells_of(lo, hi, inv);
-    int e = off[i], end = off[i + 1];
-    for (int a = r.lo[0]; a <= r.hi[0]; a++)
-        for (int b = r.lo[1]; b <= r.hi[1]; b++)
-            for (int c = r.lo[2]; c <= r.hi[2]; c++) {
-                if (e >= end) return;
-                unsigned flags = (a == r.lo[0] ? 1u : 0u) | (b == r.lo[1] ? 2u : 0u) | (c == r.lo[2] ? 4u : 0u);
-                ekey[e] = bucket_of(a, b, c, M);
-                eitem[e] = (int)((unsigned)i | (flags << 28));
-                ecell[e] = cell_key(a, b, c);
-                e++;
-            }
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (off[mid] <= e) lo = mid; else hi = mid;
+    }
+    int i = lo;
+    v3 blo, bhi;
+    box(i, blo, bhi);
+    CellRange r = cells_of(blo, bhi, inv);
+    int ny = r.hi[1] - r.lo[1] + 1, nz = r.hi[2] - r.lo[2] + 1;
+    int l = e - off[i];
+    int a = r.lo[0] + l / (ny * nz), b = r.lo[1] + (l / nz) % ny, c = r.lo[2] + l % nz;
+    unsigned flags = (a == r.lo[0] ? 1u : 0u) | (b == r.lo[1] ? 2u : 0u) | (c == r.lo[2] ? 4u : 0u);
+    ekey[e] = bucket_of(a, b, c, M);
+    eitem[e] = (int)((unsigned)i | (flags << 28));
+    ecell[e] = cell_key(a, b, c);
 }
 __global__ void k_grid_hist(int ne, const unsigned* ekey, int* bcount)
 {
@@ -208,7 +219,7 @@ static int grid_build(fipc_ctx* c, int n, const BoxFn& box)
     STCHK(ensure(ck, (size_t)ne * 2 + 2));
     unsigned long long* ecell = ck.p;
     unsigned long long* cells = ck.p + ne;
-    if (n) k_grid_fill<<<nblk(n), TPB, 0, c->s>>>(n, box, c->inv_dx, M, g.cnt.p, g.ekey.p, g.eitem.p, ecell);
+    if (ne) k_grid_fill<<<nblk(ne), TPB, 0, c->s>>>(n, ne, box, c->inv_dx, M, g.cnt.p, g.ekey.p, g.eitem.p, ecell);
     HIPCHK(hipMemsetAsync(g.bstart.p, 0, ((size_t)M + 1) * 4, c->s));
     HIPCHK(hipMemsetAsync(g.bfill.p, 0, (size_t)M * 4, c->s));
     if (ne) k_grid_hist<<<nblk(ne), TPB, 0, c->s>>>(ne, g.ekey.p, g.bstart.p);

@@ -68,8 +68,8 @@ struct Grid {
 
 // HIP-event timing of single kernels: pairs are recorded on the context
 // stream without synchronising and folded into totals when read
-#define FIPC_NTIMERS 8
-enum { TM_SPMV = 0, TM_ELASTIC, TM_CCD, TM_FIND, TM_ENERGY, TM_CONTACT, TM_PCGVEC, TM_GRID };
+#define FIPC_NTIMERS 9
+enum { TM_SPMV = 0, TM_ELASTIC, TM_CCD, TM_FIND, TM_ENERGY, TM_CONTACT, TM_PCGVEC, TM_GRID, TM_GATHER };
 struct KernelTimer {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     double total_ms = 0.0;
@@ -88,9 +88,13 @@ struct fipc_ctx {
     double dt = 0.01, dHat = 1e-3, dHat2 = 1e-6, kappa = 1.0, inv_dx = 1.0, gravity = -9.8;
     // elastic + inertia BSR (3x3 blocks, values as 9 planes of nnzb)
     int nnzb = 0;
-    DVec<int> rowptr, col, diag, slots;
+    DVec<int> rowptr, col, diag, slots, brow;
     DVec<double> val;
     DVec<float> valf;
+    // deterministic assembly: per-tet upper blocks and gradients, gathered
+    // through block -> (slot, tet) and vertex -> (corner, tet) lists
+    DVec<double> tetH, tetg;
+    DVec<int> bcptr, bcent, vcptr, vcent;
     int precision = 0;     // 0 = FP64 Hessian, 1 = FP32 Hessian / SpMV
     // contacts
     int64_t cap[NTYPES] = {0};

@@ -79,6 +79,21 @@ __device__ __forceinline__ double reduce_partials(const double* part, int n)
 }
 __device__ __forceinline__ v3 ldv(const double* a, int i) { return v3{a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
 
+// Cross-block reductions never synchronise blocks inside a kernel.  On gfx950
+// the L2 is private to each XCD, so an agent-scope release / acquire pair
+// (a __threadfence before a ticket atomic) writes back and invalidates the
+// XCD's whole L2 and destroys the cache for every other block.  Instead each
+// block stores its partial with a plain store; the next kernel on the stream
+// (the kernel boundary makes the stores visible) reduces the partials with
+// reduce_partials, redundantly in every block and in index order, so every
+// block sees the same bits.
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  For a grid whose
+// size is a multiple of 8 this maps the hardware block index to a logical one
+// such that each XCD runs a contiguous range of logical blocks, which keeps
+// neighbouring rows (and the vertices they gather) in one XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) { return (b & 7) * (nb >> 3) + (b >> 3); }
+
 // ---------------------------------------------------------------------------
 // step state
 // ---------------------------------------------------------------------------
@@ -120,22 +135,6 @@ __global__ void k_maxabs(int n, const double* p, double* part)
 // ---------------------------------------------------------------------------
 // assembly: inertia + elasticity into the BSR, contacts into per-row blocks
 // ---------------------------------------------------------------------------
-// compute_inertia (NEWTON_IPC.py:558-567): diagonal mass, rhs = -m (x - xTilde)
-__global__ void k_inertia(int nv, int nnzb, const double* __restrict__ m, const double* __restrict__ x,
-                          const double* __restrict__ xt, const int* __restrict__ diag, double* val, double* rhs,
-                          double* cdiag)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nv) return;
-    double mi = m[i];
-    int b = diag[i];
-    val[0 * nnzb + b] = mi;
-    val[4 * nnzb + b] = mi;
-    val[8 * nnzb + b] = mi;
-    for (int d = 0; d < 3; d++) rhs[3 * i + d] = -mi * (x[3 * i + d] - xt[3 * i + d]);
-    for (int k = 0; k < 9; k++) cdiag[9 * i + k] = 0.0;
-}
-
 __device__ __forceinline__ void tet_load(const int* tets, const double* IB, const double* x, int e, int (&t)[4],
                                          double (&F)[9], double (&Bm)[4][3])
 {
@@ -155,11 +154,17 @@ __device__ __forceinline__ void tet_load(const int* tets, const double* IB, cons
     }
 }
 
-// compute_elasticity (NEWTON_IPC.py:568-691): one thread per tet
-__global__ void __launch_bounds__(TPB) k_elastic(int nt, int nnzb, const int* __restrict__ tets, const double* __restrict__ IB,
+// index of the upper 3x3 block (a <= c) of a tet's 12x12 Hessian, 0..9
+__host__ __device__ constexpr int uslot(int a, int c) { return a * 4 - a * (a - 1) / 2 + (c - a); }
+
+// compute_elasticity (NEWTON_IPC.py:568-691), first pass: one thread per tet
+// writes its ten upper 3x3 blocks to tetH[(slot * nt + e) * 9 + k] and its
+// gradient to tetg[(corner * nt + e) * 3 + i]; slot-major layout so that the
+// lanes of a wave store contiguous memory
+__global__ void __launch_bounds__(TPB) k_elastic(int nt, const int* __restrict__ tets, const double* __restrict__ IB,
                                                  const double* __restrict__ vol, const double* __restrict__ la,
                                                  const double* __restrict__ mu, const double* __restrict__ x,
-                                                 const int* __restrict__ slots, double dt2, double* val, double* rhs)
+                                                 double dt2, double* __restrict__ tetH, double* __restrict__ tetg)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nt) return;
@@ -170,26 +175,77 @@ __global__ void __launch_bounds__(TPB) k_elastic(int nt, int nnzb, const int* __
     double lae = la[e], mue = mu[e];
     double P[9];
     nh_P(F, lae, mue, P);
+#pragma unroll
     for (int a = 0; a < 4; a++)
-        for (int i = 0; i < 3; i++) {
-            double gi = (P[i * 3] * Bm[a][0] + P[i * 3 + 1] * Bm[a][1] + P[i * 3 + 2] * Bm[a][2]) * sc;
-            atomicAdd(&rhs[3 * t[a] + i], -gi);
-        }
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            tetg[((size_t)a * nt + e) * 3 + i] = (P[i * 3] * Bm[a][0] + P[i * 3 + 1] * Bm[a][1] + P[i * 3 + 2] * Bm[a][2]) * sc;
     NHFactor f;
     nh_factor(F, lae, mue, f);
-    const int* sl = slots + 16 * e;
     tet_hessian_blocks(f, Bm, sc, [&](int a, int c, const double* blk) {
-        int b = sl[a * 4 + c];
+        double* o = tetH + ((size_t)uslot(a, c) * nt + e) * 9;
 #pragma unroll
-        for (int k = 0; k < 9; k++) atomicAdd(&val[k * nnzb + b], blk[k]);
-        if (c != a) {
-            int bt = sl[c * 4 + a];
+        for (int k = 0; k < 9; k++) o[k] = blk[k];
+    });
+}
+
+// second pass, one thread per BSR block: sum the contributing tet blocks in
+// a fixed order (transposed for the lower triangle), add the lumped mass on
+// diagonal blocks (compute_inertia, NEWTON_IPC.py:558-567), store the planes
+template <class T>
+__global__ void k_gather_blocks(int nnzb, const int* __restrict__ bcptr, const int* __restrict__ bcent,
+                                const double* __restrict__ tetH, const int* __restrict__ brow,
+                                const int* __restrict__ diag, const double* __restrict__ m, T* __restrict__ val)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nnzb) return;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int t = bcptr[b]; t < bcptr[b + 1]; t++) {
+        unsigned en = (unsigned)bcent[t];
+        const double* h = tetH + (size_t)(en & 0x7fffffffu) * 9;
+        double v[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[k] = h[k];
+        if (en >> 31) {
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int l = 0; l < 3; l++) atomicAdd(&val[(l * 3 + i) * nnzb + bt], blk[i * 3 + l]);
+                for (int j = 0; j < 3; j++) acc[i * 3 + j] += v[j * 3 + i];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; k++) acc[k] += v[k];
         }
-    });
+    }
+    int r = brow[b];
+    if (diag[r] == b) {
+        double mi = m[r];
+        acc[0] += mi; acc[4] += mi; acc[8] += mi;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) val[(size_t)k * nnzb + b] = (T)acc[k];
+}
+
+// rhs = -m (x - xTilde) - sum of the incident tets' gradients (rhs -= g,
+// NEWTON_IPC.py:566, :679-690); clears the contact diagonal blocks
+__global__ void k_gather_rhs(int nv, const int* __restrict__ vcptr, const int* __restrict__ vcent,
+                             const double* __restrict__ tetg, const double* __restrict__ m, const double* __restrict__ x,
+                             const double* __restrict__ xt, const unsigned char* __restrict__ dmask, double* rhs,
+                             double* cdiag)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    for (int k = 0; k < 9; k++) cdiag[9 * i + k] = 0.0;
+    if (dmask[i]) {  // rhs[D] = 0 (NEWTON_IPC.py:772)
+        rhs[3 * i] = rhs[3 * i + 1] = rhs[3 * i + 2] = 0.0;
+        return;
+    }
+    double mi = m[i];
+    double r0 = -mi * (x[3 * i] - xt[3 * i]), r1 = -mi * (x[3 * i + 1] - xt[3 * i + 1]), r2 = -mi * (x[3 * i + 2] - xt[3 * i + 2]);
+    for (int t = vcptr[i]; t < vcptr[i + 1]; t++) {
+        const double* g = tetg + (size_t)vcent[t] * 3;
+        r0 -= g[0]; r1 -= g[1]; r2 -= g[2];
+    }
+    rhs[3 * i] = r0; rhs[3 * i + 1] = r1; rhs[3 * i + 2] = r2;
 }
 
 // contact rows: a group of G lanes per row (G = 2 for PP, 4 otherwise).
@@ -221,7 +277,8 @@ template <int KIND>
 __global__ void __launch_bounds__(TPB) k_contact(int n, const int* __restrict__ rows, const double* __restrict__ x,
                                                  const double* __restrict__ x0, double dHat2, double kappa,
                                                  double* __restrict__ Hout, double* rhs, double* cdiag,
-                                                 double* Eout, double* gout, int project, int accumulate)
+                                                 double* Eout, double* gout, int project, int accumulate,
+                                                 const unsigned char* __restrict__ dmask)
 {
     using KI = KindInfo<KIND>;
     constexpr int N = KI::N, G = KI::G, K = KI::K;
@@ -287,6 +344,9 @@ __global__ void __launch_bounds__(TPB) k_contact(int n, const int* __restrict__ 
         if (L != lane) continue;
         if (accumulate) {
             int v = id[L];
+            // Dirichlet rows are replaced by the identity in the solve: skip
+            // them (a fixed plate touches thousands of rows at once)
+            if (dmask[v]) continue;
             // load_hessian_and_gradient (NEWTON_IPC.py:545-555): rhs -= g
 #pragma unroll
             for (int i = 0; i < 3; i++) atomicAdd(&rhs[3 * v + i], -g[L * 3 + i]);
@@ -306,14 +366,15 @@ __global__ void __launch_bounds__(TPB) k_contact(int n, const int* __restrict__ 
 
 // block-Jacobi preconditioner: inverse of (elastic diag block + contact diag
 // block); Dirichlet vertices get the identity (their residual is zero)
-__global__ void k_precond(int nv, int nnzb, const double* __restrict__ val, const int* __restrict__ diag,
+template <class T>
+__global__ void k_precond(int nv, int nnzb, const T* __restrict__ val, const int* __restrict__ diag,
                           const double* __restrict__ cdiag, const unsigned char* __restrict__ dmask, double* Minv)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nv) return;
     double A[9];
     int b = diag[i];
-    for (int k = 0; k < 9; k++) A[k] = val[k * nnzb + b] + cdiag[9 * i + k];
+    for (int k = 0; k < 9; k++) A[k] = (double)val[(size_t)k * nnzb + b] + cdiag[9 * i + k];
     double* o = Minv + 9 * i;
     double d = det3(A);
     if (dmask[i] || !(fabs(d) > 0.0)) {
@@ -330,8 +391,18 @@ __global__ void k_precond(int nv, int nnzb, const double* __restrict__ val, cons
 // SpMV: q += A p  (BSR part) and q += sum_rows H_r p_r (contact part)
 // ---------------------------------------------------------------------------
 #define SPMV_G 8
-// 8 lanes per block row; lanes stride over the row's blocks, then reduce.
-// Also writes per-block partials of p.q (after Dirichlet masking) when part != 0.
+// 8 lanes per block row; lanes stride over the row's blocks, then reduce with
+// shuffles.  A block handles 32 consecutive rows per pass over a contiguous
+// range of row groups (xcd_block).  When part != 0 each block also stores its
+// partial of p.q (after Dirichlet masking) in part[blockIdx.x].
+#define SPMV_ROWS (TPB / SPMV_G)
+#define SPMV_MAXB 1024
+static inline int spmv_grid(int nv)
+{
+    int groups = (nv + SPMV_ROWS - 1) / SPMV_ROWS;
+    int nb = std::min(groups, SPMV_MAXB);
+    return std::max(8, (nb + 7) & ~7);
+}
 template <class T>
 __global__ void __launch_bounds__(TPB) k_bsr_spmv(int nv, int nnzb, const int* __restrict__ rowptr,
                                                   const int* __restrict__ col, const T* __restrict__ val,
@@ -340,31 +411,38 @@ __global__ void __launch_bounds__(TPB) k_bsr_spmv(int nv, int nnzb, const int* _
                                                   const int* __restrict__ state, int it)
 {
     if (state && it > state[0]) return;
-    int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    int row = gid / SPMV_G, lane = gid % SPMV_G;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    if (row < nv) {
-        int b0 = rowptr[row], b1 = rowptr[row + 1];
-        for (int b = b0 + lane; b < b1; b += SPMV_G) {
-            int c = col[b];
-            double x0 = p[3 * c], x1 = p[3 * c + 1], x2 = p[3 * c + 2];
-            a0 += (double)val[0 * nnzb + b] * x0 + (double)val[1 * nnzb + b] * x1 + (double)val[2 * nnzb + b] * x2;
-            a1 += (double)val[3 * nnzb + b] * x0 + (double)val[4 * nnzb + b] * x1 + (double)val[5 * nnzb + b] * x2;
-            a2 += (double)val[6 * nnzb + b] * x0 + (double)val[7 * nnzb + b] * x1 + (double)val[8 * nnzb + b] * x2;
-        }
-    }
-#pragma unroll
-    for (int o = SPMV_G / 2; o > 0; o >>= 1) {
-        a0 += __shfl_xor(a0, o, 64);
-        a1 += __shfl_xor(a1, o, 64);
-        a2 += __shfl_xor(a2, o, 64);
-    }
+    int nb = gridDim.x;
+    int lb = xcd_block(blockIdx.x, nb);
+    int groups = (nv + SPMV_ROWS - 1) / SPMV_ROWS;
+    int per = (groups + nb - 1) / nb;
+    int g0 = lb * per, g1 = min(g0 + per, groups);
+    int lane = threadIdx.x % SPMV_G;
     double pq = 0.0;
-    if (row < nv && lane == 0) {
-        double y0 = q[3 * row] + a0, y1 = q[3 * row + 1] + a1, y2 = q[3 * row + 2] + a2;
-        if (dmask[row]) y0 = y1 = y2 = 0.0;
-        q[3 * row] = y0; q[3 * row + 1] = y1; q[3 * row + 2] = y2;
-        pq = p[3 * row] * y0 + p[3 * row + 1] * y1 + p[3 * row + 2] * y2;
+    for (int g = g0; g < g1; g++) {
+        int row = g * SPMV_ROWS + threadIdx.x / SPMV_G;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        if (row < nv) {
+            int b0 = rowptr[row], b1 = rowptr[row + 1];
+            for (int b = b0 + lane; b < b1; b += SPMV_G) {
+                int c = col[b];
+                double x0 = p[3 * c], x1 = p[3 * c + 1], x2 = p[3 * c + 2];
+                a0 += (double)val[0 * nnzb + b] * x0 + (double)val[1 * nnzb + b] * x1 + (double)val[2 * nnzb + b] * x2;
+                a1 += (double)val[3 * nnzb + b] * x0 + (double)val[4 * nnzb + b] * x1 + (double)val[5 * nnzb + b] * x2;
+                a2 += (double)val[6 * nnzb + b] * x0 + (double)val[7 * nnzb + b] * x1 + (double)val[8 * nnzb + b] * x2;
+            }
+        }
+#pragma unroll
+        for (int o = SPMV_G / 2; o > 0; o >>= 1) {
+            a0 += __shfl_xor(a0, o, 64);
+            a1 += __shfl_xor(a1, o, 64);
+            a2 += __shfl_xor(a2, o, 64);
+        }
+        if (row < nv && lane == 0) {
+            double y0 = q[3 * row] + a0, y1 = q[3 * row + 1] + a1, y2 = q[3 * row + 2] + a2;
+            if (dmask[row]) y0 = y1 = y2 = 0.0;
+            q[3 * row] = y0; q[3 * row + 1] = y1; q[3 * row + 2] = y2;
+            pq += p[3 * row] * y0 + p[3 * row + 1] * y1 + p[3 * row + 2] * y2;
+        }
     }
     if (part) {
         pq = block_sum(pq);
@@ -374,7 +452,8 @@ __global__ void __launch_bounds__(TPB) k_bsr_spmv(int nv, int nnzb, const int* _
 
 template <int KIND>
 __global__ void k_contact_spmv(int n, const int* __restrict__ rows, const double* __restrict__ Hc,
-                               const double* __restrict__ p, double* q, const int* __restrict__ state, int it)
+                               const double* __restrict__ p, double* q, const unsigned char* __restrict__ dmask,
+                               const int* __restrict__ state, int it)
 {
     if (state && it > state[0]) return;
     using KI = KindInfo<KIND>;
@@ -391,6 +470,7 @@ __global__ void k_contact_spmv(int n, const int* __restrict__ rows, const double
     }
     const double* Hr = Hc + (size_t)r * N * N + lane * 3 * N;
     int v = rows[W * r + lane];
+    if (dmask[v]) return;  // Dirichlet rows of q are zeroed afterwards
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         double s = 0.0;
@@ -448,17 +528,28 @@ __global__ void k_pcg_scalars(const double* part_rz, const double* part_bb, int 
         state[1] = 0;
     }
 }
-// alpha = rz / pq; x += alpha p; r -= alpha q; z = Minv r; partials of r.z, r.r
-__global__ void k_pcg_upd1(int nv, const double* part_pq, int npq, const double* scal, const unsigned char* dmask,
-                           const double* Minv, const double* p, const double* q, double* x, double* r, double* z,
-                           double* part_rz, double* part_rr, const int* state, int it)
+// scal[0|1] = r.z of the current / next iteration (by parity), scal[2] =
+// |r0|^2, scal[4] = |r|^2, scal[5] = p.q; state[0] = iteration at which the
+// test passed (done_at), state[1] = iterations run.  Both update kernels run
+// a grid-stride loop over vertices on at most PCG_MAXB blocks, so the
+// partials that the next kernel reduces stay few.
+#define PCG_MAXB 512
+static inline int pcg_grid(int nv) { return std::min(nblk(nv), PCG_MAXB); }
+// alpha = rz / pq (pq from the SpMV partials); x += alpha p; r -= alpha q;
+// z = Minv r; partials of r.z and r.r
+__global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const unsigned char* __restrict__ dmask,
+                                                  const double* __restrict__ Minv, const double* __restrict__ p,
+                                                  const double* __restrict__ q, double* x, double* r, double* z,
+                                                  const double* part_pq, int npq, double* part_rz, double* part_rr,
+                                                  const int* state, int it)
 {
     if (it > state[0]) return;
     double pq = reduce_partials(part_pq, npq);
     double alpha = scal[it & 1] / pq;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[5] = pq;
     double rz = 0.0, rr = 0.0;
-    if (i < nv && !dmask[i]) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
+        if (dmask[i]) continue;
         double ri[3];
         for (int d = 0; d < 3; d++) {
             x[3 * i + d] += alpha * p[3 * i + d];
@@ -475,26 +566,30 @@ __global__ void k_pcg_upd1(int nv, const double* part_pq, int npq, const double*
     }
     rz = block_sum(rz);
     rr = block_sum(rr);
-    if (threadIdx.x == 0) { part_rz[blockIdx.x] = rz; part_rr[blockIdx.x] = rr; }
+    if (threadIdx.x == 0) {
+        part_rz[blockIdx.x] = rz;
+        part_rr[blockIdx.x] = rr;
+    }
 }
-// beta = rz_new / rz; p = z + beta p; q = 0; convergence test on |r|^2 <= tol^2 |b|^2
-__global__ void k_pcg_upd2(int nv, const double* part_rz, const double* part_rr, int nu, double* scal,
-                           const double* z, double* p, double* q, int* state, int it, double tol2)
+// reduces r.z and r.r, runs the convergence test (block 0 records it),
+// beta = rz_new / rz; p = z + beta p; q = 0 for the next product
+__global__ void __launch_bounds__(TPB) k_pcg_upd2(int nv, double* scal, const double* __restrict__ z, double* p,
+                                                  double* q, const double* part_rz, const double* part_rr, int n1,
+                                                  int* state, int it, double tol2)
 {
     if (it > state[0]) return;
-    double rzn = reduce_partials(part_rz, nu);
-    double rr = reduce_partials(part_rr, nu);
-    double beta = rzn / scal[it & 1];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 3 * nv) {
-        p[i] = z[i] + beta * p[i];
-        q[i] = 0.0;
-    }
+    double rz_new = reduce_partials(part_rz, n1);
+    double rr = reduce_partials(part_rr, n1);
+    double beta = rz_new / scal[it & 1];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        scal[(it + 1) & 1] = rzn;
+        scal[(it + 1) & 1] = rz_new;
         scal[4] = rr;
         state[1] = it + 1;
         if (rr <= tol2 * scal[2] || !(rr == rr)) state[0] = it;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nv; i += gridDim.x * blockDim.x) {
+        p[i] = z[i] + beta * p[i];
+        q[i] = 0.0;
     }
 }
 
@@ -669,6 +764,8 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->x); release(c->x0); release(c->xn); release(c->xt); release(c->xprev); release(c->v);
     release(c->sol); release(c->rhs); release(c->target);
     release(c->rowptr); release(c->col); release(c->diag); release(c->slots); release(c->val); release(c->valf);
+    release(c->brow); release(c->tetH); release(c->tetg); release(c->bcptr); release(c->bcent); release(c->vcptr);
+    release(c->vcent);
     for (int t = 0; t < NTYPES; t++) { release(c->rows[t]); release(c->cH[t]); }
     release(c->keys_pp); release(c->keys_pe); release(c->keys_tmp); release(c->counters); release(c->cub_tmp);
     release(c->nuniq); release(c->cdiag); release(c->Minv); release(c->r); release(c->z); release(c->pp); release(c->q);
@@ -713,6 +810,40 @@ static int build_pattern(fipc_ctx* c, int nv, int nt, const int* tets)
                 int i = tets[4 * e + a], j = tets[4 * e + b];
                 slots[(size_t)e * 16 + a * 4 + b] = rowptr[i] + (int)(std::lower_bound(adj[i].begin(), adj[i].end(), j) - adj[i].begin());
             }
+    // block -> contributing (slot, tet) entries, tets in index order; bit 31
+    // marks a transposed (lower-triangle) use of the slot
+    std::vector<int> bcnt(nnzb + 1, 0);
+    for (int e = 0; e < nt; e++)
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) bcnt[slots[(size_t)e * 16 + a * 4 + b] + 1]++;
+    for (int b = 0; b < nnzb; b++) bcnt[b + 1] += bcnt[b];
+    std::vector<int> bcent((size_t)nt * 16), fill(bcnt.begin(), bcnt.end() - 1);
+    for (int e = 0; e < nt; e++)
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                int lo = std::min(a, b), hi = std::max(a, b);
+                unsigned en = (unsigned)(uslot(lo, hi) * nt + e);
+                if (a > b) en |= 0x80000000u;
+                bcent[fill[slots[(size_t)e * 16 + a * 4 + b]]++] = (int)en;
+            }
+    std::vector<int> brow(nnzb);
+    for (int i = 0; i < nv; i++)
+        for (int b = rowptr[i]; b < rowptr[i + 1]; b++) brow[b] = i;
+    // vertex -> (corner, tet) entries
+    std::vector<int> vcnt(nv + 1, 0);
+    for (int e = 0; e < nt; e++)
+        for (int a = 0; a < 4; a++) vcnt[tets[4 * e + a] + 1]++;
+    for (int i = 0; i < nv; i++) vcnt[i + 1] += vcnt[i];
+    std::vector<int> vcent((size_t)nt * 4), vfill(vcnt.begin(), vcnt.end() - 1);
+    for (int e = 0; e < nt; e++)
+        for (int a = 0; a < 4; a++) vcent[vfill[tets[4 * e + a]]++] = a * nt + e;
+    STCHK(upload(c->bcptr, bcnt.data(), bcnt.size(), c->s));
+    STCHK(upload(c->bcent, bcent.data(), bcent.size(), c->s));
+    STCHK(upload(c->brow, brow.data(), brow.size(), c->s));
+    STCHK(upload(c->vcptr, vcnt.data(), vcnt.size(), c->s));
+    STCHK(upload(c->vcent, vcent.data(), vcent.size(), c->s));
+    STCHK(ensure(c->tetH, (size_t)std::max(nt, 1) * 90));
+    STCHK(ensure(c->tetg, (size_t)std::max(nt, 1) * 12));
     c->nnzb = nnzb;
     STCHK(upload(c->rowptr, rowptr.data(), rowptr.size(), c->s));
     STCHK(upload(c->col, col.data(), col.size(), c->s));
@@ -753,8 +884,10 @@ extern "C" int fipc_upload_mesh(fipc_ctx* c, int nv, const double* x0, const dou
     HIPCHK(hipMemsetAsync(c->sol.p, 0, n3 * 8, c->s));
     HIPCHK(hipMemsetAsync(c->rhs.p, 0, n3 * 8, c->s));
     int nmax = std::max(std::max(nv, nt), 4096);
+    // block partials: k_pcg_init writes one per block of nblk(nv); the SpMV
+    // grid has at most SPMV_MAXB blocks
     STCHK(ensure(c->part_a, nblk(nmax) + 64)); STCHK(ensure(c->part_b, nblk(nmax) + 64));
-    STCHK(ensure(c->part_c, nblk(nmax) + 64));
+    STCHK(ensure(c->part_c, SPMV_MAXB + 64));
     STCHK(ensure(c->scal, 8)); STCHK(ensure(c->pcg_state, 4));
     STCHK(ensure(c->epart, nblk(nmax) * 4 + 64));
     STCHK(ensure(c->amin, 4)); STCHK(ensure(c->flag, 4)); STCHK(ensure(c->counters, 16)); STCHK(ensure(c->nuniq, 4));
@@ -879,7 +1012,7 @@ static int launch_contacts(fipc_ctx* c, int accumulate)
         int G = t == 0 ? 2 : 4;
         int nb = nblk((long)n * G);
         double* H = c->cH[t].p;
-#define LAUNCH_C(K) k_contact<K><<<nb, TPB, 0, c->s>>>(n, c->rows[t].p, c->x.p, c->x0.p, c->dHat2, c->kappa, H, c->rhs.p, c->cdiag.p, nullptr, nullptr, 1, accumulate)
+#define LAUNCH_C(K) k_contact<K><<<nb, TPB, 0, c->s>>>(n, c->rows[t].p, c->x.p, c->x0.p, c->dHat2, c->kappa, H, c->rhs.p, c->cdiag.p, nullptr, nullptr, 1, accumulate, c->dmask.p)
         switch (t) {
         case 0: LAUNCH_C(0); break;
         case 1: LAUNCH_C(1); break;
@@ -899,38 +1032,32 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
 {
     if (!c || !c->nv) return FIPC_ERR_STATE;
     c->precision = precision;
-    HIPCHK(hipMemsetAsync(c->val.p, 0, (size_t)c->nnzb * 9 * 8, c->s));
-    k_inertia<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->nnzb, c->m.p, c->x.p, c->xt.p, c->diag.p, c->val.p, c->rhs.p, c->cdiag.p);
-    HIPCHK(hipGetLastError());
     STCHK(fipc_tm_begin(c, TM_ELASTIC));
     if (c->nt)
-        k_elastic<<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->nnzb, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p, c->x.p,
-                                                  c->slots.p, c->dt * c->dt, c->val.p, c->rhs.p);
+        k_elastic<<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p, c->x.p,
+                                                  c->dt * c->dt, c->tetH.p, c->tetg.p);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_ELASTIC));
+    STCHK(fipc_tm_begin(c, TM_GATHER));
+    if (precision == 1) {
+        STCHK(ensure(c->valf, (size_t)c->nnzb * 9));
+        k_gather_blocks<float><<<nblk(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->bcent.p, c->tetH.p, c->brow.p,
+                                                                 c->diag.p, c->m.p, c->valf.p);
+    } else {
+        k_gather_blocks<double><<<nblk(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->bcent.p, c->tetH.p, c->brow.p,
+                                                                  c->diag.p, c->m.p, c->val.p);
+    }
+    k_gather_rhs<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->vcptr.p, c->vcent.p, c->tetg.p, c->m.p, c->x.p, c->xt.p,
+                                                 c->dmask.p, c->rhs.p, c->cdiag.p);
+    HIPCHK(hipGetLastError());
+    STCHK(fipc_tm_end(c, TM_GATHER));
     STCHK(fipc_tm_begin(c, TM_CONTACT));
     STCHK(launch_contacts(c, 1));
     STCHK(fipc_tm_end(c, TM_CONTACT));
-    k_precond<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->nnzb, c->val.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p);
-    HIPCHK(hipGetLastError());
-    if (precision == 1) {
-        STCHK(ensure(c->valf, (size_t)c->nnzb * 9));
-        // FP32 copy of the BSR values for the SpMV
-        extern int fipc_convert_f32(fipc_ctx*);
-        STCHK(fipc_convert_f32(c));
-    }
-    return FIPC_OK;
-}
-
-__global__ void k_to_f32(size_t n, const double* a, float* b)
-{
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = (float)a[i];
-}
-int fipc_convert_f32(fipc_ctx* c)
-{
-    size_t n = (size_t)c->nnzb * 9;
-    k_to_f32<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, c->s>>>(n, c->val.p, c->valf.p);
+    if (precision == 1)
+        k_precond<float><<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->nnzb, c->valf.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p);
+    else
+        k_precond<double><<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->nnzb, c->val.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p);
     HIPCHK(hipGetLastError());
     return FIPC_OK;
 }
@@ -943,7 +1070,7 @@ static int spmv(fipc_ctx* c, const double* p, double* q, double* part, const int
         if (!n) continue;
         int K = t == 0 ? 2 : (t == 1 ? 3 : 4);
         int nb = nblk((long)n * K);
-#define LAUNCH_S(KK) k_contact_spmv<KK><<<nb, TPB, 0, c->s>>>(n, c->rows[t].p, c->cH[t].p, p, q, state, it)
+#define LAUNCH_S(KK) k_contact_spmv<KK><<<nb, TPB, 0, c->s>>>(n, c->rows[t].p, c->cH[t].p, p, q, c->dmask.p, state, it)
         switch (t) {
         case 0: LAUNCH_S(0); break;
         case 1: LAUNCH_S(1); break;
@@ -956,12 +1083,14 @@ static int spmv(fipc_ctx* c, const double* p, double* q, double* part, const int
 #undef LAUNCH_S
         HIPCHK(hipGetLastError());
     }
-    int nb = nblk((long)c->nv * SPMV_G);
+    int nb = spmv_grid(c->nv);
     STCHK(fipc_tm_begin(c, TM_SPMV));
     if (c->precision == 1)
-        k_bsr_spmv<float><<<nb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->valf.p, p, q, c->dmask.p, part, state, it);
+        k_bsr_spmv<float><<<nb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->valf.p, p, q, c->dmask.p, part,
+                                                 state, it);
     else
-        k_bsr_spmv<double><<<nb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->val.p, p, q, c->dmask.p, part, state, it);
+        k_bsr_spmv<double><<<nb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->val.p, p, q, c->dmask.p, part,
+                                                  state, it);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_SPMV));
     return FIPC_OK;
@@ -983,7 +1112,6 @@ extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double
     if (!c || !c->nv) return FIPC_ERR_STATE;
     int nv = c->nv;
     int nbv = nblk(nv);
-    int nbs = nblk((long)nv * SPMV_G);
     // x0 = prescribed Dirichlet increments; q = A x0
     k_pcg_x0<<<nbv, TPB, 0, c->s>>>(nv, c->xt.p, c->x.p, c->dmask.p, c->sol.p, c->q.p);
     HIPCHK(hipGetLastError());
@@ -995,16 +1123,17 @@ extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double
     HIPCHK(hipGetLastError());
     int it = 0, state[2] = {0, 0};
     const int chunk = 16;
+    const int nsp = spmv_grid(nv), nu = pcg_grid(nv);
     double tol2 = tol * tol;
     while (it < maxit) {
         int end = std::min(maxit, it + chunk);
         for (; it < end; it++) {
             STCHK(spmv(c, c->pp.p, c->q.p, c->part_c.p, c->pcg_state.p, it));
             STCHK(fipc_tm_begin(c, TM_PCGVEC));
-            k_pcg_upd1<<<nbv, TPB, 0, c->s>>>(nv, c->part_c.p, nbs, c->scal.p, c->dmask.p, c->Minv.p, c->pp.p, c->q.p,
-                                               c->sol.p, c->r.p, c->z.p, c->part_a.p, c->part_b.p, c->pcg_state.p, it);
-            k_pcg_upd2<<<nblk(nv * 3), TPB, 0, c->s>>>(nv, c->part_a.p, c->part_b.p, nbv, c->scal.p, c->z.p, c->pp.p,
-                                                        c->q.p, c->pcg_state.p, it, tol2);
+            k_pcg_upd1<<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->dmask.p, c->Minv.p, c->pp.p, c->q.p, c->sol.p, c->r.p,
+                                              c->z.p, c->part_c.p, nsp, c->part_a.p, c->part_b.p, c->pcg_state.p, it);
+            k_pcg_upd2<<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->z.p, c->pp.p, c->q.p, c->part_a.p, c->part_b.p, nu,
+                                              c->pcg_state.p, it, tol2);
             STCHK(fipc_tm_end(c, TM_PCGVEC));
         }
         HIPCHK(hipGetLastError());
@@ -1031,19 +1160,18 @@ extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double
 extern "C" int fipc_energy(fipc_ctx* c, double* E, double* parts)
 {
     if (!c || !c->nv || !E) return FIPC_ERR_STATE;
-    STCHK(fipc_tm_begin(c, TM_ENERGY));
-    int nbv = nblk(c->nv), nbt = nblk(std::max(c->nt, 1));
-    double* ep = c->epart.p;
-    k_energy_inertia<<<nbv, TPB, 0, c->s>>>(c->nv, c->m.p, c->x.p, c->xt.p, ep);
-    if (c->nt) k_energy_elastic<<<nbt, TPB, 0, c->s>>>(c->nt, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p, c->x.p, c->dt * c->dt, ep + nbv);
-    HIPCHK(hipGetLastError());
+    int nbv = nblk(c->nv), nbt = c->nt ? nblk(c->nt) : 0;
     std::vector<int> nbc(NTYPES, 0);
-    size_t off = nbv + (c->nt ? nbt : 0);
+    size_t off = nbv + nbt;
     size_t total = off;
     for (int t = 0; t < NTYPES; t++) { nbc[t] = c->count[t] ? nblk(c->count[t]) : 0; total += nbc[t]; }
+    // the buffer may grow with the contact counts: size it before any launch writes into it
     STCHK(ensure(c->epart, total + 8));
-    ep = c->epart.p;
-    if (c->nt == 0) nbt = 0;
+    double* ep = c->epart.p;
+    STCHK(fipc_tm_begin(c, TM_ENERGY));
+    k_energy_inertia<<<nbv, TPB, 0, c->s>>>(c->nv, c->m.p, c->x.p, c->xt.p, ep);
+    if (nbt) k_energy_elastic<<<nbt, TPB, 0, c->s>>>(c->nt, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p, c->x.p, c->dt * c->dt, ep + nbv);
+    HIPCHK(hipGetLastError());
     size_t o = off;
     for (int t = 0; t < NTYPES; t++) {
         if (!nbc[t]) continue;
@@ -1123,8 +1251,15 @@ extern "C" int fipc_get_bsr(fipc_ctx* c, int* nnzb, int* rowptr, int* col, doubl
     if (col) HIPCHK(hipMemcpyAsync(col, c->col.p, (size_t)c->nnzb * 4, hipMemcpyDeviceToHost, c->s));
     if (val) {
         std::vector<double> planes((size_t)c->nnzb * 9);
-        HIPCHK(hipMemcpyAsync(planes.data(), c->val.p, planes.size() * 8, hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
+        if (c->precision == 1) {
+            std::vector<float> pf(planes.size());
+            HIPCHK(hipMemcpyAsync(pf.data(), c->valf.p, pf.size() * 4, hipMemcpyDeviceToHost, c->s));
+            HIPCHK(hipStreamSynchronize(c->s));
+            for (size_t i = 0; i < pf.size(); i++) planes[i] = pf[i];
+        } else {
+            HIPCHK(hipMemcpyAsync(planes.data(), c->val.p, planes.size() * 8, hipMemcpyDeviceToHost, c->s));
+            HIPCHK(hipStreamSynchronize(c->s));
+        }
         for (size_t b = 0; b < (size_t)c->nnzb; b++)
             for (int k = 0; k < 9; k++) val[b * 9 + k] = planes[(size_t)k * c->nnzb + b];
     }
@@ -1140,11 +1275,11 @@ extern "C" int fipc_enable_timing(fipc_ctx* c, int on)
     for (auto& t : c->tm) { t.total_ms = 0; t.launches = 0; }
     return FIPC_OK;
 }
-extern "C" int fipc_kernel_times(fipc_ctx* c, double* ms, long* launches)
+extern "C" int fipc_kernel_times(fipc_ctx* c, int n, double* ms, long* launches)
 {
-    if (!c) return FIPC_ERR_ARG;
+    if (!c || n < 0) return FIPC_ERR_ARG;
     STCHK(fipc_tm_fold(c));
-    for (int i = 0; i < FIPC_NTIMERS; i++) {
+    for (int i = 0; i < n && i < FIPC_NTIMERS; i++) {
         if (ms) ms[i] = c->tm[i].total_ms;
         if (launches) launches[i] = c->tm[i].launches;
     }
@@ -1182,7 +1317,7 @@ extern "C" int fipc_batch_constraint(int type, int n, const double* x, const dou
     HIPCHK(hipMemcpy(dx0, x0 ? x0 : x, npts * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(drows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
     int nb = nblk((long)n * G);
-#define LAUNCH_B(KK) k_contact<KK><<<nb, TPB>>>(n, drows, dx, dx0, dHat2, kappa, dH, nullptr, nullptr, dE, dg, project, 0)
+#define LAUNCH_B(KK) k_contact<KK><<<nb, TPB>>>(n, drows, dx, dx0, dHat2, kappa, dH, nullptr, nullptr, dE, dg, project, 0, nullptr)
     switch (type) {
     case 0: LAUNCH_B(0); break;
     case 1: LAUNCH_B(1); break;

@@ -185,6 +185,8 @@ template <int R> struct RGrad { v3 g[R]; m3 H[R][R]; double v; };
 // N(u, w) = |u x w|^2 : value, gradient, Hessian in (u, w)
 FI void cross2_derivs(v3 u, v3 w, RGrad<2>& o)
 {
+    // derivative-only arithmetic: contraction into FMA is allowed here
+#pragma clang fp contract(fast)
     v3 c = cross(u, w);
     o.v = n2(c);
     o.g[0] = 2.0 * cross(w, c);
@@ -199,6 +201,8 @@ FI void cross2_derivs(v3 u, v3 w, RGrad<2>& o)
 // d = (r . (e1 x e2))^2 / |e1 x e2|^2 in (r, e1, e2)
 FI void tri_derivs(v3 r, v3 e1, v3 e2, RGrad<3>& o)
 {
+    // derivative-only arithmetic: contraction into FMA is allowed here
+#pragma clang fp contract(fast)
     v3 n = cross(e1, e2);
     double s = dot(r, n);
     RGrad<2> q;
@@ -230,6 +234,8 @@ FI void tri_derivs(v3 r, v3 e1, v3 e2, RGrad<3>& o)
 // d = |u x w|^2 / |w - u|^2 in (u = e0 - p, w = e1 - p)
 FI void pe_derivs(v3 u, v3 w, RGrad<2>& o)
 {
+    // derivative-only arithmetic: contraction into FMA is allowed here
+#pragma clang fp contract(fast)
     RGrad<2> Nn;
     cross2_derivs(u, w, Nn);
     v3 e = w - u;
@@ -254,6 +260,8 @@ FI void pe_derivs(v3 u, v3 w, RGrad<2>& o)
 template <int K, int R>
 FI void expand(const RGrad<R>& y, const int (&C)[K][R], double* g, double* H)
 {
+    // derivative-only arithmetic: contraction into FMA is allowed here
+#pragma clang fp contract(fast)
 #pragma unroll
     for (int k = 0; k < K; k++) {
         v3 s = mk(0, 0, 0);
@@ -431,6 +439,8 @@ __host__ __device__ constexpr int pk(int N, int i, int j)
 template <int N, int R>
 FI void jacobi_eig(double* A, double (&V)[R][N], int row0, int max_sweeps)
 {
+    // derivative-only arithmetic: contraction into FMA is allowed here
+#pragma clang fp contract(fast)
 #pragma unroll
     for (int r = 0; r < R; r++)
 #pragma unroll
@@ -483,6 +493,8 @@ FI void jacobi_eig(double* A, double (&V)[R][N], int row0, int max_sweeps)
 template <int N>
 FI void make_pd_full(double* M)
 {
+    // derivative-only arithmetic: contraction into FMA is allowed here
+#pragma clang fp contract(fast)
     double A[N * (N + 1) / 2];
     for (int i = 0; i < N; i++) for (int j = i; j < N; j++) A[pk(N, i, j)] = 0.5 * (M[i * N + j] + M[j * N + i]);
     double V[N][N];
@@ -522,6 +534,8 @@ FI void make_pd2(double* M)
 // singular values in descending order, the last one signed by det F.
 FI void svd3(const double* F, double* U, double* s, double* V)
 {
+    // derivative-only arithmetic: contraction into FMA is allowed here
+#pragma clang fp contract(fast)
     double A[6];
     for (int i = 0; i < 3; i++)
         for (int j = i; j < 3; j++) A[pk(3, i, j)] = F[i] * F[j] + F[3 + i] * F[3 + j] + F[6 + i] * F[6 + j];
@@ -584,6 +598,8 @@ struct NHFactor {
 };
 FI void nh_factor(const double* F, double la, double mu, NHFactor& o)
 {
+    // derivative-only arithmetic: contraction into FMA is allowed here
+#pragma clang fp contract(fast)
     double s[3];
     svd3(F, o.U, s, o.V);
     double lp = log(s[0] * s[1] * s[2]), middle = mu - la * lp;
@@ -614,6 +630,8 @@ FI void nh_factor(const double* F, double la, double mu, NHFactor& o)
 template <class Emit>
 FI void tet_hessian_blocks(const NHFactor& f, const double (&Bm)[4][3], double sc, Emit emit)
 {
+    // derivative-only arithmetic: contraction into FMA is allowed here
+#pragma clang fp contract(fast)
     // bv[a][b] = sum_j Bm[a][j] V[j][b]
     double bv[4][3];
 #pragma unroll
@@ -652,17 +670,23 @@ FI void tet_hessian_blocks(const NHFactor& f, const double (&Bm)[4][3], double s
         }
 #pragma unroll
         for (int c = a; c < 4; c++) {
+            // z_l[(c,j)] = bv[c][lb] U[j][la] for l = la*3+lb
+            double Zc[3][9];
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+#pragma unroll
+                for (int la = 0; la < 3; la++)
+#pragma unroll
+                    for (int lb = 0; lb < 3; lb++) Zc[j][la * 3 + lb] = bv[c][lb] * f.U[j * 3 + la];
             double blk[9];
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int l = 0; l < 3; l++) {
+                for (int j = 0; j < 3; j++) {
                     double s = 0.0;
 #pragma unroll
-                    for (int ka = 0; ka < 3; ka++)
-#pragma unroll
-                        for (int kb = 0; kb < 3; kb++) s += Y[i][ka * 3 + kb] * bv[c][kb] * f.U[l * 3 + ka];
-                    blk[i * 3 + l] = s;
+                    for (int l = 0; l < 9; l++) s += Y[i][l] * Zc[j][l];
+                    blk[i * 3 + j] = s;
                 }
             emit(a, c, blk);
         }

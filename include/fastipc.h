@@ -164,12 +164,13 @@ int fipc_get_bsr(fipc_ctx* c, int* nnzb, int* rowptr, int* col, double* val);
 /* y = A x with the assembled Hessian (elastic BSR + contact rows), Dirichlet rows zeroed */
 int fipc_spmv(fipc_ctx* c, const double* x, double* y);
 /* timing of the hot kernels with HIP events on the context stream, recorded
- * without synchronisation while enabled and folded when read:
- * ms[8] / launches[8] = BSR SpMV, elastic assembly, CCD narrow+broad query,
- * constraint/collision query, energy, contact assembly, PCG vector updates,
- * grid build (phase wall time on the stream) */
+ * without synchronisation while enabled and folded when read.  Classes:
+ * 0 BSR SpMV, 1 per-tet elastic pass, 2 CCD query, 3 constraint / collision
+ * query, 4 energy, 5 contact assembly, 6 PCG vector updates, 7 grid build
+ * (phase time on the stream), 8 assembly gathers.  Writes the first n (<= 9)
+ * totals (ms) and launch counts. */
 int fipc_enable_timing(fipc_ctx* c, int on);
-int fipc_kernel_times(fipc_ctx* c, double* ms, long* launches);
+int fipc_kernel_times(fipc_ctx* c, int n, double* ms, long* launches);
 int fipc_sync(fipc_ctx* c);
 
 #ifdef __cplusplus

@@ -721,81 +721,55 @@ static void find_intervals(const double* cin, int n, int pos, ivlset* s)
     for (int i = 0; i + 1 < nr; i++) check_interval(r[i], r[i + 1], c, deg, s, pos);
     check_interval(r[nr - 1], 1.0, c, deg, s, pos);
 }
-/* vector polynomial of degree 1: x + t v */
-typedef struct { double c[3][3]; int deg; } vpoly; /* c[power][xyz], deg <= 2 */
-static void vp_lin(const double* x, const double* v, vpoly* o)
+/* Polynomial coefficients of the three CTCD sign tests along the linear
+ * trajectory y(t) = x + t v.  The expressions are written out term by term
+ * (cross products of the start/velocity parts, then dot products) in the
+ * same order as the product library, so both strict-IEEE implementations
+ * round identically on degenerate (parallel, touching) configurations. */
+static void lin_cross3(const double* x2, const double* v2, const double* x3, const double* v3, double n[3][3])
 {
-    memset(o, 0, sizeof *o);
-    for (int d = 0; d < 3; d++) { o->c[0][d] = x[d]; o->c[1][d] = v[d]; }
-    o->deg = 1;
+    double a[3], b[3];
+    vcross(x2, x3, n[0]);
+    vcross(x2, v3, a); vcross(v2, x3, b); vadd(a, b, n[1]);
+    vcross(v2, v3, n[2]);
 }
-static void vp_cross(const vpoly* a, const vpoly* b, vpoly* o)
+static void triple3(const double* x1, const double* v1, double n[3][3], double* f)
 {
-    vpoly r;
-    memset(&r, 0, sizeof r);
-    r.deg = a->deg + b->deg;
-    for (int i = 0; i <= a->deg; i++)
-        for (int j = 0; j <= b->deg; j++) {
-            double t[3];
-            vcross(a->c[i], b->c[j], t);
-            for (int d = 0; d < 3; d++) r.c[i + j][d] += t[d];
-        }
-    *o = r;
-}
-static void vp_dot(const vpoly* a, const vpoly* b, double* out, int* deg)
-{
-    int dg = a->deg + b->deg;
-    for (int k = 0; k <= dg; k++) out[k] = 0;
-    for (int i = 0; i <= a->deg; i++)
-        for (int j = 0; j <= b->deg; j++) out[i + j] += vdot(a->c[i], b->c[j]);
-    *deg = dg;
-}
-static void p_mul(const double* a, int da, const double* b, int db, double* o)
-{
-    double r[8] = {0};
-    for (int i = 0; i <= da; i++) for (int j = 0; j <= db; j++) r[i + j] += a[i] * b[j];
-    for (int i = 0; i <= da + db; i++) o[i] = r[i];
+    f[0] = vdot(x1, n[0]);
+    f[1] = vdot(x1, n[1]) + vdot(v1, n[0]);
+    f[2] = vdot(x1, n[2]) + vdot(v1, n[1]);
+    f[3] = vdot(v1, n[2]);
 }
 /* sign-set of (x1 + t v1) . ((x2 + t v2) x (x3 + t v3)) >= 0 (CTCD.cpp:201-215) */
 static void plane_set(const double* x1, const double* x2, const double* x3, const double* v1, const double* v2, const double* v3, ivlset* s)
 {
-    vpoly a, b, c, bc;
-    vp_lin(x1, v1, &a); vp_lin(x2, v2, &b); vp_lin(x3, v3, &c);
-    vp_cross(&b, &c, &bc);
-    double p[8]; int dg;
-    vp_dot(&a, &bc, p, &dg);
-    find_intervals(p, 3, 1, s);
+    double n[3][3], f[4];
+    lin_cross3(x2, v2, x3, v3, n);
+    triple3(x1, v1, n, f);
+    find_intervals(f, 3, 1, s);
 }
 /* distance-to-plane^2 <= minD2 set (CTCD.cpp:217-243):
  * (x1.(x2 x x3))^2 - minD2 |x2 x x3|^2 <= 0 along the linear trajectory */
 static void dist_set(const double* x1, const double* x2, const double* x3, const double* v1, const double* v2, const double* v3, double minD2, ivlset* s)
 {
-    vpoly a, b, c, bc;
-    vp_lin(x1, v1, &a); vp_lin(x2, v2, &b); vp_lin(x3, v3, &c);
-    vp_cross(&b, &c, &bc);
-    double f[8], n2[8], ff[8]; int df, dn;
-    vp_dot(&a, &bc, f, &df);
-    vp_dot(&bc, &bc, n2, &dn);
-    p_mul(f, 3, f, 3, ff);
-    for (int i = 0; i <= 4; i++) ff[i] -= minD2 * n2[i];
-    find_intervals(ff, 6, 0, s);
+    double n[3][3], f[4], p[7] = {0, 0, 0, 0, 0, 0, 0};
+    lin_cross3(x2, v2, x3, v3, n);
+    triple3(x1, v1, n, f);
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) p[i + j] += f[i] * f[j];
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) p[i + j] -= minD2 * vdot(n[i], n[j]);
+    find_intervals(p, 6, 0, s);
 }
 /* barycentric side set (CTCD.cpp:165-199): with e0 = x1 + t v1,
  * e1 = x2 + t v2 and q = x3 + t v3 the polynomial is
  *   (e1.e0)(q.e0) - |e0|^2 (q.e1)  >= 0 */
 static void bary_set(const double* x1, const double* x2, const double* x3, const double* v1, const double* v2, const double* v3, ivlset* s)
 {
-    vpoly a, b, c;
-    vp_lin(x1, v1, &a); vp_lin(x2, v2, &b); vp_lin(x3, v3, &c);
-    double aa[8], ba[8], cb[8], ca[8], t1[8], t2[8], p[8];
-    int d;
-    vp_dot(&a, &a, aa, &d);   /* |e0|^2 */
-    vp_dot(&b, &a, ba, &d);   /* e1.e0 */
-    vp_dot(&c, &b, cb, &d);   /* q.e1 */
-    vp_dot(&c, &a, ca, &d);   /* q.e0 */
-    p_mul(ba, 2, ca, 2, t1);
-    p_mul(aa, 2, cb, 2, t2);
-    for (int i = 0; i <= 4; i++) p[i] = t1[i] - t2[i];
+    double aa[3] = {vdot(x1, x1), 2 * vdot(x1, v1), vdot(v1, v1)};
+    double ba[3] = {vdot(x2, x1), vdot(x2, v1) + vdot(v2, x1), vdot(v2, v1)};
+    double cb[3] = {vdot(x3, x2), vdot(x3, v2) + vdot(v3, x2), vdot(v3, v2)};
+    double ca[3] = {vdot(x3, x1), vdot(x3, v1) + vdot(v3, x1), vdot(v3, v1)};
+    double p[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) p[i + j] += ba[i] * ca[j] - aa[i] * cb[j];
     find_intervals(p, 4, 1, s);
 }
 static int ivl_overlap(ivl a, ivl b) { return !(a.l > b.u || b.l > a.u); }

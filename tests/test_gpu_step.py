@@ -70,7 +70,10 @@ def test_constraints_energy_gradient_hessian(scene_fn, steps):
         assert abs(a - b) <= 1e-10 * abs(b) or abs(a - b) < 1e-14 * abs(Eo)
     g.compute_hessian_and_gradient()
     o.assemble()
-    assert np.abs(g.vector(2).ravel() - o.rhs).max() <= 1e-10 * np.abs(o.rhs).max()
+    D = np.repeat(o.dmask, 3)
+    ro = o.rhs.copy()
+    ro[D] = 0.0  # rhs[D] = 0 (NEWTON_IPC.py:772); the GPU does not assemble fixed rows
+    assert np.abs(g.vector(2).ravel() - ro).max() <= 1e-10 * np.abs(ro).max()
     # 2) deformed copy of that state: every tet strained, Hessian products
     x = o.x.copy()
     free = ~sc.dirichlet_mask
@@ -81,9 +84,10 @@ def test_constraints_energy_gradient_hessian(scene_fn, steps):
     o.find_constraints()
     g.compute_hessian_and_gradient()
     o.assemble()
-    assert np.abs(g.vector(2).ravel() - o.rhs).max() <= 1e-10 * np.abs(o.rhs).max()
+    ro = o.rhs.copy()
+    ro[D] = 0.0
+    assert np.abs(g.vector(2).ravel() - ro).max() <= 1e-10 * np.abs(ro).max()
     A = o.matrix()
-    D = np.repeat(o.dmask, 3)
     for _ in range(3):
         v = rng.normal(size=3 * o.nv)
         v[D] = 0.0
@@ -121,12 +125,19 @@ def test_filters_match_oracle():
     assert g.check_collision() == o.check_collision() == 1
 
 
-@pytest.mark.parametrize("scene_fn,steps", [(S.tiny_pair, 4), (lambda: S.two_cubes(4), 12)])
-def test_steps_match_oracle(scene_fn, steps):
+@pytest.mark.parametrize("scene_fn,steps,amp", [(S.tiny_pair, 4, 0.001), (lambda: S.two_cubes(4), 12, 0.004)])
+def test_steps_match_oracle(scene_fn, steps, amp):
+    """Whole time steps from the same pre-strained state (every tet away from
+    F = I, where the reference's 2x2 make_pd jumps) with the Newton tolerance
+    tightened 1e3 times: positions agree within 1e-6 of the bbox diagonal."""
     sc = scene_fn()
-    sc.tol_rel *= 1e-4
+    sc.tol_rel *= 1e-3
     g = NewtonIPC(sc, pcg_tol=1e-12)
     o = OracleIPC(sc)
+    x = _deformed(sc, amp=amp, seed=2)
+    o.x = np.ascontiguousarray(x.copy())
+    assert o.check_collision() == 0
+    g.set_state(x)
     for f in range(steps):
         sg = g.step()
         so = o.step()
