@@ -30,7 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TIMER_NAMES = ("bsr_spmv", "elastic_tet_pass", "ccd_query", "contact_query", "energy", "contact_assembly",
+TIMER_NAMES = ("spmv", "elastic_tet_pass", "ccd_query", "contact_query", "energy", "contact_assembly",
                "pcg_vector", "grid_build", "assembly_gather")
 
 
@@ -47,9 +47,12 @@ def algorithmic_bytes(sim, precision):
     _lib.call("fipc_get_bsr", sim.ctx, C.byref(nnzb), None, None, None)
     nnzb = nnzb.value
     sa = 4 if precision == 1 else 8
+    width = (2, 3, 4, 4, 4, 4, 4)
+    contact = sum(n * ((3 * w) ** 2 * 8 + 4 * w) for n, w in zip(sim.counts, width))
     return {
-        # values + column index per block, row pointers, x read and y written
-        "bsr_spmv": nnzb * (9 * sa + 4) + 4 * (nv + 1) + 2 * 3 * nv * 8,
+        # BSR values + column index per block, row pointers, p read, q written,
+        # contact rows (dense H_r + indices)
+        "spmv": nnzb * (9 * sa + 4) + 4 * (nv + 1) + 2 * 3 * nv * 8 + contact,
         # per tet: 4 indices, D_m^-1, vol, lambda, mu, 16 slots; per vertex x;
         # the BSR values written once and the gradient
         "elastic_assembly": nt * (16 + 72 + 24 + 64) + nv * 24 + nnzb * 9 * 8 + 3 * nv * 8,
@@ -85,6 +88,14 @@ def run_gpu(args):
             raise RuntimeError("intersection after a step")
     _lib.call("fipc_sync", sim.ctx)
     wall = time.perf_counter() - t0
+    if args.traffic_probe:
+        # counter-calibration run: a plain device copy of known size, no event timing
+        cms = C.c_double()
+        _lib.call("fipc_stream_copy", sim.ctx, 256 << 20, C.byref(cms))
+        log(f"traffic probe: copied {256 << 20} bytes device to device in {cms.value:.3f} ms "
+            f"({2 * (256 << 20) / cms.value / 1e6:.0f} GB/s)")
+        sim.close()
+        return None
     # one extra step with kernel timing on
     _lib.call("fipc_enable_timing", sim.ctx, 1)
     st = sim.step()
@@ -170,12 +181,17 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-warmup", type=int, default=6)
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="counter-calibration run for rocprofv3 --pmc: the timed steps, then a 256 MiB "
+                         "device copy instead of the timing step; prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design note, not an implementation)", file=sys.stderr)
         sys.exit(2)
     g = run_gpu(args)
+    if g is None:
+        return
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(args)
     line = {
         "metric": "sim steps/s at 1M tets, warm-start+FP32 vs plain FP64 GPU at equal residual",

@@ -59,6 +59,7 @@ _SIGS = {
     "fipc_enable_timing": [_VP, C.c_int],
     "fipc_kernel_times": [_VP, C.c_int, _P, C.POINTER(C.c_long)],
     "fipc_sync": [_VP],
+    "fipc_stream_copy": [_VP, C.c_long, _P],
     "fipc_batch_constraint": [C.c_int, C.c_int, _P, _P, _D, _D, C.c_int, _P, _P, _P],
     "fipc_batch_tet": [C.c_int, _P, _P, _P, _P, _P, _P, _P, _P],
     "fipc_batch_ccd": [C.c_int, C.c_int, _P, _P, _D, _P, _P],

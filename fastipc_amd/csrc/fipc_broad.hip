@@ -6,9 +6,11 @@
 // (cell size = 1 / inv_dx, as the reference's hash).  Entries are grouped by
 // a hashed bucket; each entry keeps its exact 63-bit cell key and three bits
 // telling whether the cell is the item's lowest cell along x, y, z.  A query
-// walks the cells of its own box in chunks of QCHUNK cells (one thread per
-// chunk, so huge boxes do not serialise a wave) and accepts an entry only in
-// the first cell common to both boxes, so every pair is tested exactly once.
+// runs one thread per (query box, cell) pair, so huge boxes do not serialise
+// a wave, and accepts an entry only in the first cell common to both boxes,
+// so every pair is tested exactly once.  Builds and queries size their
+// buffers from capacities and never wait for the host; each operation
+// synchronises once at its end and is redone when a capacity was exceeded.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 #include <algorithm>
@@ -18,7 +20,6 @@
 using namespace fipc;
 
 #define TPB 256
-#define QCHUNK 32
 static inline int nblk(long n, int t = TPB) { return (int)((n + t - 1) / t); }
 
 __device__ __forceinline__ v3 ldv(const double* a, int i) { return v3{a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
@@ -82,7 +83,10 @@ struct BoxFn {
     }
 };
 
-__global__ void k_grid_count(int n, BoxFn box, double inv, int* cnt, int* overflow)
+#define GRID_MAXB 2048
+static inline int stride_grid(long n) { return std::max(1, std::min(nblk(n), GRID_MAXB)); }
+
+__global__ void k_grid_count(int n, BoxFn box, double inv, int* cnt, int* flag)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > n) return;
@@ -90,51 +94,66 @@ __global__ void k_grid_count(int n, BoxFn box, double inv, int* cnt, int* overfl
     v3 lo, hi;
     box(i, lo, hi);
     long long c = ncells(cells_of(lo, hi, inv));
-    if (c > (1 << 24) || c < 0) { atomicOr(overflow, 1); c = 0; }
+    if (c > (1 << 24) || c < 0) { atomicOr(flag, GF_HUGE); c = 0; }
     cnt[i] = (int)c;
 }
-// one thread per grid entry: the owning item is found by binary search in
-// the scanned counts, so an item covering thousands of cells (a long plate
-// edge) does not serialise a wave
-__global__ void k_grid_fill(int n, int ne, BoxFn box, double inv, unsigned M, const int* __restrict__ off,
-                            unsigned* ekey, int* eitem, unsigned long long* ecell)
+// Grid-stride over the entries (ne = off[n], scanned on the device, so the
+// host never waits for it).  A count above the capacity flags a retry and
+// records the count needed.  The owning item of an entry is found by binary
+// search in the scanned counts, so an item covering thousands of cells (a
+// long plate edge) does not serialise a wave.
+__global__ void k_grid_fill(int n, int cap, BoxFn box, double inv, unsigned M, const int* __restrict__ off,
+                            unsigned* ekey, int* eitem, unsigned long long* ecell, int* flag)
 {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
-    int lo = 0, hi = n;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (off[mid] <= e) lo = mid; else hi = mid;
+    int ne = off[n];
+    if (ne > cap || ne < 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            atomicOr(flag, ne < 0 ? GF_HUGE : GF_RETRY);
+            atomicMax(flag + 2, ne);
+        }
+        return;
     }
-    int i = lo;
-    v3 blo, bhi;
-    box(i, blo, bhi);
-    CellRange r = cells_of(blo, bhi, inv);
-    int ny = r.hi[1] - r.lo[1] + 1, nz = r.hi[2] - r.lo[2] + 1;
-    int l = e - off[i];
-    int a = r.lo[0] + l / (ny * nz), b = r.lo[1] + (l / nz) % ny, c = r.lo[2] + l % nz;
-    unsigned flags = (a == r.lo[0] ? 1u : 0u) | (b == r.lo[1] ? 2u : 0u) | (c == r.lo[2] ? 4u : 0u);
-    ekey[e] = bucket_of(a, b, c, M);
-    eitem[e] = (int)((unsigned)i | (flags << 28));
-    ecell[e] = cell_key(a, b, c);
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
+        int lo = 0, hi = n;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (off[mid] <= e) lo = mid; else hi = mid;
+        }
+        int i = lo;
+        v3 blo, bhi;
+        box(i, blo, bhi);
+        CellRange r = cells_of(blo, bhi, inv);
+        int ny = r.hi[1] - r.lo[1] + 1, nz = r.hi[2] - r.lo[2] + 1;
+        int l = e - off[i];
+        int a = r.lo[0] + l / (ny * nz), b = r.lo[1] + (l / nz) % ny, c = r.lo[2] + l % nz;
+        unsigned flags = (a == r.lo[0] ? 1u : 0u) | (b == r.lo[1] ? 2u : 0u) | (c == r.lo[2] ? 4u : 0u);
+        ekey[e] = bucket_of(a, b, c, M);
+        eitem[e] = (int)((unsigned)i | (flags << 28));
+        ecell[e] = cell_key(a, b, c);
+    }
 }
-__global__ void k_grid_hist(int ne, const unsigned* ekey, int* bcount)
+__global__ void k_grid_hist(const int* __restrict__ nep, int cap, const unsigned* ekey, int* bcount)
 {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < ne) atomicAdd(&bcount[ekey[e]], 1);
+    int ne = *nep;
+    if (ne > cap) return;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x)
+        atomicAdd(&bcount[ekey[e]], 1);
 }
-__global__ void k_grid_scatter(int ne, const unsigned* ekey, const int* eitem, const unsigned long long* ecell,
-                               const int* bstart, int* bfill, int* items, unsigned long long* cells)
+__global__ void k_grid_scatter(const int* __restrict__ nep, int cap, const unsigned* ekey, const int* eitem,
+                               const unsigned long long* ecell, const int* bstart, int* bfill, int* items,
+                               unsigned long long* cells)
 {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne) return;
-    unsigned k = ekey[e];
-    int pos = bstart[k] + atomicAdd(&bfill[k], 1);
-    items[pos] = eitem[e];
-    cells[pos] = ecell[e];
+    int ne = *nep;
+    if (ne > cap) return;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
+        unsigned k = ekey[e];
+        int pos = bstart[k] + atomicAdd(&bfill[k], 1);
+        items[pos] = eitem[e];
+        cells[pos] = ecell[e];
+    }
 }
-// number of QCHUNK-cell chunks of every query box
-__global__ void k_query_chunks(int n, BoxFn box, double inv, int* cnt, int* overflow)
+// number of cells of every query box
+__global__ void k_query_cells(int n, BoxFn box, double inv, int* cnt, int* flag)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > n) return;
@@ -142,37 +161,41 @@ __global__ void k_query_chunks(int n, BoxFn box, double inv, int* cnt, int* over
     v3 lo, hi;
     box(i, lo, hi);
     long long c = ncells(cells_of(lo, hi, inv));
-    if (c > (1 << 26) || c < 0) { atomicOr(overflow, 1); c = 0; }
-    cnt[i] = (int)((c + QCHUNK - 1) / QCHUNK);
+    if (c > (1 << 24) || c < 0) { atomicOr(flag, GF_HUGE); c = 0; }
+    cnt[i] = (int)c;
 }
 
-// generic query: visit(q, item) is called once per (query, item) pair whose
-// cell ranges overlap
+// Generic query, one thread per (query, cell) pair, grid-stride: visit(q,
+// item) is called once per (query, item) pair whose cell ranges overlap,
+// in the first cell common to both.  Does nothing when the grid build
+// flagged a retry (the caller then redoes the whole operation).
 template <class Visit>
-__global__ void __launch_bounds__(TPB) k_grid_query(int nq, int nchunks, const int* __restrict__ qoff, BoxFn qbox,
-                                                    double inv, unsigned M, const int* __restrict__ bstart,
+__global__ void __launch_bounds__(TPB) k_grid_query(int nq, const int* __restrict__ qoff, BoxFn qbox, double inv,
+                                                    unsigned M, const int* __restrict__ bstart,
                                                     const int* __restrict__ items,
-                                                    const unsigned long long* __restrict__ cells, Visit visit)
+                                                    const unsigned long long* __restrict__ cells, int* flag,
+                                                    Visit visit)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nchunks) return;
-    // query owning chunk t
-    int lo = 0, hi = nq;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (qoff[mid] <= t) lo = mid; else hi = mid;
+    if (flag[0] & (GF_RETRY | GF_HUGE)) return;
+    int total = qoff[nq];
+    if (total < 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(flag, GF_HUGE);
+        return;
     }
-    int q = lo, chunk = t - qoff[q];
-    v3 blo, bhi;
-    qbox(q, blo, bhi);
-    CellRange r = cells_of(blo, bhi, inv);
-    int ny = r.hi[1] - r.lo[1] + 1, nz = r.hi[2] - r.lo[2] + 1;
-    long long total = ncells(r);
-    long long l0 = (long long)chunk * QCHUNK, l1 = l0 + QCHUNK < total ? l0 + QCHUNK : total;
-    for (long long l = l0; l < l1; l++) {
-        int a = r.lo[0] + (int)(l / ((long long)ny * nz));
-        int b = r.lo[1] + (int)((l / nz) % ny);
-        int c = r.lo[2] + (int)(l % nz);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        int lo = 0, hi = nq;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (qoff[mid] <= t) lo = mid; else hi = mid;
+        }
+        int q = lo, l = t - qoff[q];
+        v3 blo, bhi;
+        qbox(q, blo, bhi);
+        CellRange r = cells_of(blo, bhi, inv);
+        int ny = r.hi[1] - r.lo[1] + 1, nz = r.hi[2] - r.lo[2] + 1;
+        int a = r.lo[0] + l / (ny * nz);
+        int b = r.lo[1] + (l / nz) % ny;
+        int c = r.lo[2] + l % nz;
         unsigned long long key = cell_key(a, b, c);
         unsigned bk = bucket_of(a, b, c, M);
         int e0 = bstart[bk], e1 = bstart[bk + 1];
@@ -187,88 +210,84 @@ __global__ void __launch_bounds__(TPB) k_grid_query(int nq, int nchunks, const i
     }
 }
 
-// build the grid over n items
+static int scan_inplace(fipc_ctx* c, int* d, int n)
+{
+    size_t bytes = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, d, d, n, c->s));
+    if (bytes > c->cub_tmp.cap) STCHK(ensure(c->cub_tmp, bytes));
+    bytes = c->cub_tmp.cap;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, d, d, n, c->s));
+    return FIPC_OK;
+}
+
+// Build the grid over n items without waiting for the host: buffers are
+// sized by the capacity, overflow is flagged in flag[0] (see GF_RETRY).
 static int grid_build(fipc_ctx* c, int n, const BoxFn& box)
 {
     Grid& g = c->grid;
     STCHK(fipc_tm_begin(c, TM_GRID));
+    int cap = g.cap;
+    unsigned M = 1024;
+    while (M < (unsigned)cap * 2u && M < (1u << 30)) M <<= 1;
+    g.M = M;
     STCHK(ensure(g.cnt, (size_t)n + 2));
-    HIPCHK(hipMemsetAsync(c->flag.p, 0, 4, c->s));
+    STCHK(ensure(g.ekey, cap)); STCHK(ensure(g.eitem, cap)); STCHK(ensure(g.items, cap));
+    STCHK(ensure(g.ecell, cap)); STCHK(ensure(g.cells, cap));
+    STCHK(ensure(g.bstart, (size_t)M + 1)); STCHK(ensure(g.bfill, M));
     k_grid_count<<<nblk(n + 1), TPB, 0, c->s>>>(n, box, c->inv_dx, g.cnt.p, c->flag.p);
     HIPCHK(hipGetLastError());
-    size_t bytes = 0;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, g.cnt.p, g.cnt.p, n + 1, c->s));
-    if (bytes > c->cub_tmp.cap) STCHK(ensure(c->cub_tmp, bytes));
-    bytes = c->cub_tmp.cap;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, g.cnt.p, g.cnt.p, n + 1, c->s));
-    int hv[2];
-    HIPCHK(hipMemcpyAsync(&hv[0], g.cnt.p + n, 4, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(&hv[1], c->flag.p, 4, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    if (hv[1]) return FIPC_ERR_CAPACITY;
-    int ne = hv[0];
-    g.nentries = ne;
-    unsigned M = 1024;
-    while (M < (unsigned)ne * 2u && M < (1u << 30)) M <<= 1;
-    g.M = M;
-    STCHK(ensure(g.ekey, ne)); STCHK(ensure(g.eitem, ne));
-    STCHK(ensure(g.bstart, (size_t)M + 1)); STCHK(ensure(g.bfill, M)); STCHK(ensure(g.items, ne));
-    static_assert(sizeof(unsigned long long) == 8, "");
-    // cell keys: entries and sorted copy share one allocation of 2*ne
-    DVec<unsigned long long>& ck = c->keys_tmp;
-    STCHK(ensure(ck, (size_t)ne * 2 + 2));
-    unsigned long long* ecell = ck.p;
-    unsigned long long* cells = ck.p + ne;
-    if (ne) k_grid_fill<<<nblk(ne), TPB, 0, c->s>>>(n, ne, box, c->inv_dx, M, g.cnt.p, g.ekey.p, g.eitem.p, ecell);
+    STCHK(scan_inplace(c, g.cnt.p, n + 1));
+    const int* nep = g.cnt.p + n;
+    int nb = stride_grid(cap);
+    k_grid_fill<<<nb, TPB, 0, c->s>>>(n, cap, box, c->inv_dx, M, g.cnt.p, g.ekey.p, g.eitem.p, g.ecell.p, c->flag.p);
     HIPCHK(hipMemsetAsync(g.bstart.p, 0, ((size_t)M + 1) * 4, c->s));
     HIPCHK(hipMemsetAsync(g.bfill.p, 0, (size_t)M * 4, c->s));
-    if (ne) k_grid_hist<<<nblk(ne), TPB, 0, c->s>>>(ne, g.ekey.p, g.bstart.p);
+    k_grid_hist<<<nb, TPB, 0, c->s>>>(nep, cap, g.ekey.p, g.bstart.p);
     HIPCHK(hipGetLastError());
-    bytes = 0;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, g.bstart.p, g.bstart.p, M + 1, c->s));
-    if (bytes > c->cub_tmp.cap) STCHK(ensure(c->cub_tmp, bytes));
-    bytes = c->cub_tmp.cap;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, g.bstart.p, g.bstart.p, M + 1, c->s));
-    if (ne) k_grid_scatter<<<nblk(ne), TPB, 0, c->s>>>(ne, g.ekey.p, g.eitem.p, ecell, g.bstart.p, g.bfill.p, g.items.p, cells);
+    STCHK(scan_inplace(c, g.bstart.p, M + 1));
+    k_grid_scatter<<<nb, TPB, 0, c->s>>>(nep, cap, g.ekey.p, g.eitem.p, g.ecell.p, g.bstart.p, g.bfill.p, g.items.p,
+                                         g.cells.p);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_GRID));
     return FIPC_OK;
 }
 
-// chunk offsets of nq query boxes in qoff (n+1), returns the chunk count
-static int query_prep(fipc_ctx* c, int nq, const BoxFn& qbox, DVec<int>& qoff, int* nchunks)
-{
-    STCHK(ensure(qoff, (size_t)nq + 2));
-    HIPCHK(hipMemsetAsync(c->flag.p, 0, 4, c->s));
-    k_query_chunks<<<nblk(nq + 1), TPB, 0, c->s>>>(nq, qbox, c->inv_dx, qoff.p, c->flag.p);
-    HIPCHK(hipGetLastError());
-    size_t bytes = 0;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, qoff.p, qoff.p, nq + 1, c->s));
-    if (bytes > c->cub_tmp.cap) STCHK(ensure(c->cub_tmp, bytes));
-    bytes = c->cub_tmp.cap;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, qoff.p, qoff.p, nq + 1, c->s));
-    int hv[2];
-    HIPCHK(hipMemcpyAsync(&hv[0], qoff.p + nq, 4, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(&hv[1], c->flag.p, 4, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    if (hv[1]) return FIPC_ERR_CAPACITY;
-    *nchunks = hv[0];
-    return FIPC_OK;
-}
-
+// query the last grid with nq boxes; no host synchronisation
 template <class Visit>
 static int run_query(fipc_ctx* c, int nq, const BoxFn& qbox, Visit visit, int timer)
 {
-    if (nq == 0 || c->grid.nentries == 0) return FIPC_OK;
-    int nchunks = 0;
-    STCHK(query_prep(c, nq, qbox, c->grid.qoff, &nchunks));
-    if (!nchunks) return FIPC_OK;
+    if (nq == 0) return FIPC_OK;
     Grid& g = c->grid;
+    STCHK(ensure(g.qoff, (size_t)nq + 2));
     STCHK(fipc_tm_begin(c, timer));
-    k_grid_query<<<nblk(nchunks), TPB, 0, c->s>>>(nq, nchunks, c->grid.qoff.p, qbox, c->inv_dx, g.M, g.bstart.p, g.items.p,
-                                                   c->keys_tmp.p + g.nentries, visit);
+    k_query_cells<<<nblk(nq + 1), TPB, 0, c->s>>>(nq, qbox, c->inv_dx, g.qoff.p, c->flag.p);
+    HIPCHK(hipGetLastError());
+    STCHK(scan_inplace(c, g.qoff.p, nq + 1));
+    k_grid_query<<<GRID_MAXB, TPB, 0, c->s>>>(nq, g.qoff.p, qbox, c->inv_dx, g.M, g.bstart.p, g.items.p, g.cells.p,
+                                                c->flag.p, visit);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, timer));
+    return FIPC_OK;
+}
+
+// start of a broad-phase operation: clear the status word and needed count
+static int bp_begin(fipc_ctx* c)
+{
+    HIPCHK(hipMemsetAsync(c->flag.p, 0, 12, c->s));
+    return FIPC_OK;
+}
+// after the operation's synchronisation: error, retry (capacity grown) or done
+static int bp_check(fipc_ctx* c, const int* hflag, bool* retry)
+{
+    if (hflag[0] & GF_HUGE) return FIPC_ERR_CAPACITY;
+    *retry = false;
+    if (hflag[0] & GF_RETRY) {
+        long need = (long)hflag[2];
+        long cap = need + need / 4 + 4096;
+        if (cap > (1L << 30)) return FIPC_ERR_CAPACITY;
+        c->grid.cap = (int)cap;
+        *retry = true;
+    }
     return FIPC_OK;
 }
 
@@ -359,44 +378,42 @@ struct VisitEE {
     }
 };
 
-__global__ void k_decode_pp(int n, const unsigned long long* k, int* rows)
+// decode the unique sorted keys (count read on the device) into rows
+__global__ void k_decode_pp(const int* __restrict__ np, const unsigned long long* k, int* rows)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    unsigned long long v = k[i];
-    rows[2 * i] = (int)(v >> 21);
-    rows[2 * i + 1] = (int)(v & ((1ull << 21) - 1));
+    int n = *np;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        unsigned long long v = k[i];
+        rows[2 * i] = (int)(v >> 21);
+        rows[2 * i + 1] = (int)(v & ((1ull << 21) - 1));
+    }
 }
-__global__ void k_decode_pe(int n, const unsigned long long* k, int* rows)
+__global__ void k_decode_pe(const int* __restrict__ np, const unsigned long long* k, int* rows)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    unsigned long long v = k[i], m = (1ull << 21) - 1;
-    rows[3 * i] = (int)(v >> 42);
-    rows[3 * i + 1] = (int)((v >> 21) & m);
-    rows[3 * i + 2] = (int)(v & m);
+    int n = *np;
+    unsigned long long m = (1ull << 21) - 1;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        unsigned long long v = k[i];
+        rows[3 * i] = (int)(v >> 42);
+        rows[3 * i + 1] = (int)((v >> 21) & m);
+        rows[3 * i + 2] = (int)(v & m);
+    }
 }
 
-
-static int dedup(fipc_ctx* c, unsigned long long* keys, int n, int end_bit, int* nuniq)
+// np.unique order: radix sort of the packed keys, unique into uniq; the
+// unique count is left on the device (*d_n)
+static int dedup(fipc_ctx* c, const unsigned long long* keys, int n, int end_bit, unsigned long long* sorted,
+                 unsigned long long* uniq, int* d_n)
 {
-    if (n == 0) { *nuniq = 0; return FIPC_OK; }
-    unsigned long long* sorted = c->keys_tmp.p;
-    STCHK(ensure(c->keys_tmp, (size_t)n * 2 + 2));
-    sorted = c->keys_tmp.p;
-    unsigned long long* uniq = sorted + n;
     size_t b1 = 0, b2 = 0;
     HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, b1, keys, sorted, n, 0, end_bit, c->s));
-    HIPCHK(hipcub::DeviceSelect::Unique(nullptr, b2, sorted, uniq, c->nuniq.p, n, c->s));
+    HIPCHK(hipcub::DeviceSelect::Unique(nullptr, b2, sorted, uniq, d_n, n, c->s));
     size_t need = std::max(b1, b2);
     if (need > c->cub_tmp.cap) STCHK(ensure(c->cub_tmp, need));
     b1 = c->cub_tmp.cap;
     HIPCHK(hipcub::DeviceRadixSort::SortKeys(c->cub_tmp.p, b1, keys, sorted, n, 0, end_bit, c->s));
     b2 = c->cub_tmp.cap;
-    HIPCHK(hipcub::DeviceSelect::Unique(c->cub_tmp.p, b2, sorted, uniq, c->nuniq.p, n, c->s));
-    HIPCHK(hipMemcpyAsync(nuniq, c->nuniq.p, 4, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    HIPCHK(hipMemcpyAsync(keys, uniq, (size_t)*nuniq * 8, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipcub::DeviceSelect::Unique(c->cub_tmp.p, b2, sorted, uniq, d_n, n, c->s));
     return FIPC_OK;
 }
 
@@ -407,8 +424,11 @@ static int bits_for(int n)
     return b;
 }
 
-static int find_once(fipc_ctx* c, int* counts)
+// one pass of both searches; raw counts and the status word come back with
+// the single synchronisation
+static int find_once(fipc_ctx* c, int* counts, int* hflag)
 {
+    STCHK(bp_begin(c));
     HIPCHK(hipMemsetAsync(c->counters.p, 0, NTYPES * 4, c->s));
     RowSink sink;
     sink.cnt = c->counters.p;
@@ -417,7 +437,6 @@ static int find_once(fipc_ctx* c, int* counts)
     sink.kpe = c->keys_pe.p;
     // PT: points inserted with +-dHat, triangles query their box
     BoxFn pts{c->x.p, nullptr, c->bp.p, 0, c->dHat};
-    // point boxes index bp directly: kind 0 reads idx[i]
     STCHK(grid_build(c, c->nbp, pts));
     BoxFn tris{c->x.p, nullptr, c->bt.p, 2, 0.0};
     VisitPT vpt{c->x.p, c->bp.p, c->bt.p, c->dHat, c->dHat2, sink};
@@ -429,6 +448,7 @@ static int find_once(fipc_ctx* c, int* counts)
     VisitEE vee{c->x.p, c->x0.p, c->be.p, c->dHat, c->dHat2, sink};
     STCHK(run_query(c, c->nbe, qedges, vee, TM_FIND));
     HIPCHK(hipMemcpyAsync(counts, c->counters.p, NTYPES * 4, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(hflag, c->flag.p, 12, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     return FIPC_OK;
 }
@@ -437,36 +457,50 @@ extern "C" int fipc_find_constraints(fipc_ctx* c, int* counts_out)
 {
     if (!c || !c->nv) return FIPC_ERR_STATE;
     int raw[NTYPES];
-    for (int attempt = 0; attempt < 4; attempt++) {
+    for (int attempt = 0;; attempt++) {
+        if (attempt == 6) return FIPC_ERR_CAPACITY;
         STCHK(ensure(c->keys_pp, c->cap[0]));
         STCHK(ensure(c->keys_pe, c->cap[1]));
         for (int t = 2; t < NTYPES; t++) STCHK(ensure(c->rows[t], c->cap[t] * 4));
-        STCHK(find_once(c, raw));
+        int hflag[3];
+        STCHK(find_once(c, raw, hflag));
         bool grow = false;
+        STCHK(bp_check(c, hflag, &grow));
+        if (grow) continue;
         for (int t = 0; t < NTYPES; t++)
             if (raw[t] > c->cap[t]) { c->cap[t] = (int64_t)raw[t] * 2; grow = true; }
         if (!grow) break;
-        if (attempt == 3) return FIPC_ERR_CAPACITY;
     }
     for (int t = 0; t < NTYPES; t++) c->raw_count[t] = raw[t];
     int bits = bits_for(c->nv);
-    int npp = 0, npe = 0;
-    STCHK(dedup(c, c->keys_pp.p, raw[0], std::min(64, 21 + bits), &npp));
-    STCHK(dedup(c, c->keys_pe.p, raw[1], std::min(64, 42 + bits), &npe));
-    STCHK(ensure(c->rows[0], (size_t)std::max(npp, 1) * 2));
-    STCHK(ensure(c->rows[1], (size_t)std::max(npe, 1) * 3));
-    if (npp) k_decode_pp<<<nblk(npp), TPB, 0, c->s>>>(npp, c->keys_pp.p, c->rows[0].p);
-    if (npe) k_decode_pe<<<nblk(npe), TPB, 0, c->s>>>(npe, c->keys_pe.p, c->rows[1].p);
+    // sorted and unique keys of PP then PE share keys_tmp
+    size_t npp_raw = raw[0], npe_raw = raw[1];
+    STCHK(ensure(c->keys_tmp, 2 * (npp_raw + npe_raw) + 4));
+    unsigned long long* kt = c->keys_tmp.p;
+    STCHK(ensure(c->rows[0], std::max<size_t>(npp_raw, 1) * 2));
+    STCHK(ensure(c->rows[1], std::max<size_t>(npe_raw, 1) * 3));
+    HIPCHK(hipMemsetAsync(c->nuniq.p, 0, 8, c->s));
+    if (npp_raw) {
+        STCHK(dedup(c, c->keys_pp.p, raw[0], std::min(64, 21 + bits), kt, kt + npp_raw, c->nuniq.p));
+        k_decode_pp<<<stride_grid(npp_raw), TPB, 0, c->s>>>(c->nuniq.p, kt + npp_raw, c->rows[0].p);
+    }
+    if (npe_raw) {
+        unsigned long long* k2 = kt + 2 * npp_raw;
+        STCHK(dedup(c, c->keys_pe.p, raw[1], std::min(64, 42 + bits), k2, k2 + npe_raw, c->nuniq.p + 1));
+        k_decode_pe<<<stride_grid(npe_raw), TPB, 0, c->s>>>(c->nuniq.p + 1, k2 + npe_raw, c->rows[1].p);
+    }
     HIPCHK(hipGetLastError());
-    c->count[0] = npp;
-    c->count[1] = npe;
+    int nu[2];
+    HIPCHK(hipMemcpyAsync(nu, c->nuniq.p, 8, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->count[0] = nu[0];
+    c->count[1] = nu[1];
     for (int t = 2; t < NTYPES; t++) c->count[t] = raw[t];
     for (int t = 0; t < NTYPES; t++) {
         int N = 3 * (t == 0 ? 2 : (t == 1 ? 3 : 4));
         STCHK(ensure(c->cH[t], (size_t)std::max(c->count[t], 1) * N * N));
         if (counts_out) counts_out[t] = c->count[t];
     }
-    HIPCHK(hipStreamSynchronize(c->s));
     return FIPC_OK;
 }
 
@@ -481,16 +515,40 @@ extern "C" int fipc_get_constraints(fipc_ctx* c, int type, int* rows_out)
 }
 
 // ---- CCD filters ----------------------------------------------------------
-__device__ __forceinline__ void amin_d(unsigned long long* a, double v)
+// Candidate pairs: the grid walk applies the moving-box test of the reference
+// and the CCD lower bound (a pair whose bound lb = (1 - eta) d / maxDisp is
+// at least 1 keeps toc = 1, wrapper.cpp:254-328), and appends the rest; the
+// narrow phase (CTCD) then runs one thread per candidate, so its long,
+// data-dependent root finding neither stalls the grid walk nor unbalances
+// its waves.
+// One atomic per wavefront: the active lanes that append take consecutive slots.
+__device__ __forceinline__ int wave_append(int* cnt)
 {
-    atomicMin(a, (unsigned long long)__double_as_longlong(v));
+    unsigned long long mask = __ballot(1);
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    int lane = __lane_id();
+    int base = 0;
+    if (lane == leader) base = atomicAdd(cnt, __popcll(mask));
+    base = __shfl(base, leader, 64);
+    unsigned long long below = lane == 0 ? 0ull : (mask & (~0ull >> (64 - lane)));
+    return base + __popcll(below);
 }
+struct PairSink {
+    int* cnt;
+    int cap;
+    int2* pairs;
+    __device__ __forceinline__ void push(int a, int b) const
+    {
+        int s = wave_append(cnt);
+        if (s < cap) pairs[s] = make_int2(a, b);
+    }
+};
 // compute_filter_3D_PT (NEWTON_IPC.py:377-406); query = triangle, item = point
-struct VisitCCDPT {
+struct CandCCDPT {
     const double *x, *p;
     const int *bp, *bt;
     double dHat, eta;
-    unsigned long long* amin;
+    PairSink out;
     __device__ __forceinline__ void operator()(int tri, int pi) const
     {
         int q = bp[pi], t0 = bt[3 * tri], t1 = bt[3 * tri + 1], t2 = bt[3 * tri + 2];
@@ -503,17 +561,20 @@ struct VisitCCDPT {
         v3 mnt = vmin(vmin(T0, T0 + d0), vmin(vmin(T1, T1 + d1), vmin(T2, T2 + d2)));
         if (!(mnp.x < mxt.x + dHat && mnp.y < mxt.y + dHat && mnp.z < mxt.z + dHat)) return;
         if (!(mnt.x - dHat < mxp.x && mnt.y - dHat < mxp.y && mnt.z - dHat < mxp.z)) return;
+        // the lower bound of pt_ccd, same expressions
+        double maxDisp = sqrt(n2(dP)) + sqrt(fmax(fmax(n2(d0), n2(d1)), n2(d2)));
+        if (maxDisp == 0) return;
         double d2c = PT_dist2(P, T0, T1, T2, PT_type(P, T0, T1, T2));
-        double toc = pt_ccd(P, T0, T1, T2, dP, d0, d1, d2, eta, d2c);
-        if (toc < 1.0) amin_d(amin, toc);
+        if (!((1 - eta) * sqrt(d2c) / maxDisp < 1)) return;
+        out.push(tri, pi);
     }
 };
 // compute_filter_3D_EE (NEWTON_IPC.py:409-440); query edge i, item edge j
-struct VisitCCDEE {
+struct CandCCDEE {
     const double *x, *p;
     const int* be;
     double dHat, eta;
-    unsigned long long* amin;
+    PairSink out;
     __device__ __forceinline__ void operator()(int i, int j) const
     {
         if (!(i < j)) return;
@@ -526,37 +587,108 @@ struct VisitCCDEE {
         v3 mxb = vmax(vmax(B0, B0 + dB0), vmax(B1, B1 + dB1)), mnb = vmin(vmin(B0, B0 + dB0), vmin(B1, B1 + dB1));
         if (!(mna.x < mxb.x + dHat && mna.y < mxb.y + dHat && mna.z < mxb.z + dHat)) return;
         if (!(mnb.x - dHat < mxa.x && mnb.y - dHat < mxa.y && mnb.z - dHat < mxa.z)) return;
+        // the lower bound of ee_ccd, same expressions
+        double maxDisp = sqrt(fmax(n2(dA0), n2(dA1))) + sqrt(fmax(n2(dB0), n2(dB1)));
+        if (maxDisp == 0) return;
+        double d2c = EE_dist2(A0, A1, B0, B1, EE_type(A0, A1, B0, B1));
+        if (!((1 - eta) * sqrt(d2c) / maxDisp < 1)) return;
+        out.push(i, j);
+    }
+};
+__device__ __forceinline__ void amin_d(unsigned long long* a, double v)
+{
+    atomicMin(a, (unsigned long long)__double_as_longlong(v));
+}
+// narrow phase over the candidates (count read on the device, capped at cap:
+// an overflowing count makes the host redo the filter with more room)
+__global__ void __launch_bounds__(TPB) k_ccd_pt(const int* __restrict__ cnt, int cap, const int2* __restrict__ pairs,
+                                                const double* __restrict__ x, const double* __restrict__ p,
+                                                const int* __restrict__ bp, const int* __restrict__ bt, double eta,
+                                                unsigned long long* amin)
+{
+    int n = min(*cnt, cap);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int2 pr = pairs[i];
+        int q = bp[pr.y], t0 = bt[3 * pr.x], t1 = bt[3 * pr.x + 1], t2 = bt[3 * pr.x + 2];
+        v3 P = ldv(x, q), T0 = ldv(x, t0), T1 = ldv(x, t1), T2 = ldv(x, t2);
+        v3 dP = ldv(p, q), d0 = ldv(p, t0), d1 = ldv(p, t1), d2 = ldv(p, t2);
+        double d2c = PT_dist2(P, T0, T1, T2, PT_type(P, T0, T1, T2));
+        double toc = pt_ccd(P, T0, T1, T2, dP, d0, d1, d2, eta, d2c);
+        if (toc < 1.0) amin_d(amin, toc);
+    }
+}
+__global__ void __launch_bounds__(TPB) k_ccd_ee(const int* __restrict__ cnt, int cap, const int2* __restrict__ pairs,
+                                                const double* __restrict__ x, const double* __restrict__ p,
+                                                const int* __restrict__ be, double eta, unsigned long long* amin)
+{
+    int n = min(*cnt, cap);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int2 pr = pairs[i];
+        int a0 = be[2 * pr.x], a1 = be[2 * pr.x + 1], b0 = be[2 * pr.y], b1 = be[2 * pr.y + 1];
+        v3 A0 = ldv(x, a0), A1 = ldv(x, a1), B0 = ldv(x, b0), B1 = ldv(x, b1);
+        v3 dA0 = ldv(p, a0), dA1 = ldv(p, a1), dB0 = ldv(p, b0), dB1 = ldv(p, b1);
         double d2c = EE_dist2(A0, A1, B0, B1, EE_type(A0, A1, B0, B1));
         double toc = ee_ccd(A0, A1, B0, B1, dA0, dA1, dB0, dB1, eta, d2c);
         if (toc < 1.0) amin_d(amin, toc);
     }
-};
+}
 
 extern int fipc_inversion_alpha(fipc_ctx* c, unsigned long long* dst);
 
+// compute_intersection_free_step_size (NEWTON_IPC.py:456-468)
 extern "C" int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* alpha, double* parts)
 {
     if (!c || !c->nv || !alpha) return FIPC_ERR_STATE;
-    double one = 1.0;
-    unsigned long long init[3];
-    for (int i = 0; i < 3; i++) memcpy(&init[i], &one, 8);
-    HIPCHK(hipMemcpyAsync(c->amin.p, init, 24, hipMemcpyHostToDevice, c->s));
-    BoxFn pts{c->x.p, c->sol.p, c->bp.p, 0, c->dHat};
-    STCHK(grid_build(c, c->nbp, pts));
-    BoxFn tris{c->x.p, c->sol.p, c->bt.p, 2, 0.0};
-    VisitCCDPT vpt{c->x.p, c->sol.p, c->bp.p, c->bt.p, c->dHat, eta, c->amin.p};
-    STCHK(run_query(c, c->nbt, tris, vpt, TM_CCD));
-    BoxFn edges{c->x.p, c->sol.p, c->be.p, 1, 0.0};
-    STCHK(grid_build(c, c->nbe, edges));
-    BoxFn qedges{c->x.p, c->sol.p, c->be.p, 1, c->dHat};
-    VisitCCDEE vee{c->x.p, c->sol.p, c->be.p, c->dHat, eta, c->amin.p + 1};
-    STCHK(run_query(c, c->nbe, qedges, vee, TM_CCD));
-    if (inversion) STCHK(fipc_inversion_alpha(c, c->amin.p + 2));
-    unsigned long long out[3];
-    HIPCHK(hipMemcpyAsync(out, c->amin.p, 24, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
     double a[3];
-    memcpy(a, out, 24);
+    for (int attempt = 0;; attempt++) {
+        if (attempt == 6) return FIPC_ERR_CAPACITY;
+        int cap = c->ccd_cap;
+        STCHK(ensure(c->ccd_pairs[0], cap)); STCHK(ensure(c->ccd_pairs[1], cap));
+        STCHK(bp_begin(c));
+        int* pc = c->counters.p + 8;
+        HIPCHK(hipMemsetAsync(pc, 0, 8, c->s));
+        double one = 1.0;
+        unsigned long long init[3];
+        for (int i = 0; i < 3; i++) memcpy(&init[i], &one, 8);
+        HIPCHK(hipMemcpyAsync(c->amin.p, init, 24, hipMemcpyHostToDevice, c->s));
+        BoxFn pts{c->x.p, c->sol.p, c->bp.p, 0, c->dHat};
+        STCHK(grid_build(c, c->nbp, pts));
+        BoxFn tris{c->x.p, c->sol.p, c->bt.p, 2, 0.0};
+        CandCCDPT vpt{c->x.p, c->sol.p, c->bp.p, c->bt.p, c->dHat, eta, PairSink{pc, cap, c->ccd_pairs[0].p}};
+        STCHK(run_query(c, c->nbt, tris, vpt, TM_CCD));
+        STCHK(fipc_tm_begin(c, TM_CCD));
+        k_ccd_pt<<<stride_grid(cap), TPB, 0, c->s>>>(pc, cap, c->ccd_pairs[0].p, c->x.p, c->sol.p, c->bp.p, c->bt.p, eta,
+                                                     c->amin.p);
+        HIPCHK(hipGetLastError());
+        STCHK(fipc_tm_end(c, TM_CCD));
+        BoxFn edges{c->x.p, c->sol.p, c->be.p, 1, 0.0};
+        STCHK(grid_build(c, c->nbe, edges));
+        BoxFn qedges{c->x.p, c->sol.p, c->be.p, 1, c->dHat};
+        CandCCDEE vee{c->x.p, c->sol.p, c->be.p, c->dHat, eta, PairSink{pc + 1, cap, c->ccd_pairs[1].p}};
+        STCHK(run_query(c, c->nbe, qedges, vee, TM_CCD));
+        STCHK(fipc_tm_begin(c, TM_CCD));
+        k_ccd_ee<<<stride_grid(cap), TPB, 0, c->s>>>(pc + 1, cap, c->ccd_pairs[1].p, c->x.p, c->sol.p, c->be.p, eta,
+                                                     c->amin.p + 1);
+        HIPCHK(hipGetLastError());
+        STCHK(fipc_tm_end(c, TM_CCD));
+        if (inversion) STCHK(fipc_inversion_alpha(c, c->amin.p + 2));
+        unsigned long long out[3];
+        int hflag[3], hc[2];
+        HIPCHK(hipMemcpyAsync(out, c->amin.p, 24, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipMemcpyAsync(hflag, c->flag.p, 12, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipMemcpyAsync(hc, pc, 8, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        bool retry = false;
+        STCHK(bp_check(c, hflag, &retry));
+        int need = std::max(hc[0], hc[1]);
+        if (need > cap) {
+            c->ccd_cap = need + need / 2 + 1024;
+            retry = true;
+        }
+        if (retry) continue;
+        memcpy(a, out, 24);
+        break;
+    }
     if (parts) { parts[0] = a[0]; parts[1] = a[1]; parts[2] = a[2]; }
     *alpha = std::min(std::min(a[0], a[1]), a[2]);
     return FIPC_OK;
@@ -575,20 +707,27 @@ struct VisitCollide {
     }
 };
 
+// check_collision (NEWTON_IPC.py:855-886)
 extern "C" int fipc_check_collision(fipc_ctx* c, int* hit)
 {
     if (!c || !c->nv || !hit) return FIPC_ERR_STATE;
-    BoxFn edges{c->x.p, nullptr, c->be.p, 1, c->dHat};
-    STCHK(grid_build(c, c->nbe, edges));
-    HIPCHK(hipMemsetAsync(c->flag.p + 1, 0, 4, c->s));
-    BoxFn tris{c->x.p, nullptr, c->bt.p, 2, 0.0};
-    VisitCollide v{c->x.p, c->be.p, c->bt.p, c->flag.p + 1};
-    STCHK(run_query(c, c->nbt, tris, v, TM_FIND));
-    int h = 0;
-    HIPCHK(hipMemcpyAsync(&h, c->flag.p + 1, 4, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    *hit = h;
-    return FIPC_OK;
+    for (int attempt = 0;; attempt++) {
+        if (attempt == 6) return FIPC_ERR_CAPACITY;
+        STCHK(bp_begin(c));
+        BoxFn edges{c->x.p, nullptr, c->be.p, 1, c->dHat};
+        STCHK(grid_build(c, c->nbe, edges));
+        BoxFn tris{c->x.p, nullptr, c->bt.p, 2, 0.0};
+        VisitCollide v{c->x.p, c->be.p, c->bt.p, c->flag.p + 1};
+        STCHK(run_query(c, c->nbt, tris, v, TM_FIND));
+        int hflag[3];
+        HIPCHK(hipMemcpyAsync(hflag, c->flag.p, 12, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        bool retry = false;
+        STCHK(bp_check(c, hflag, &retry));
+        if (retry) continue;
+        *hit = hflag[1];
+        return FIPC_OK;
+    }
 }
 
 // ---- batched CCD for tests -------------------------------------------------

@@ -54,16 +54,23 @@ static inline void release(DVec<T>& v)
 }
 
 // hashed uniform grid used by every broad phase
+// Broad-phase status word (flag[0]) bits: GF_RETRY = a grid had more entries
+// than its capacity (the needed count is in flag[2]; the caller grows the
+// capacity and redoes the operation), GF_HUGE = one box spans more cells than
+// the limit (an error).
+#define GF_RETRY 1
+#define GF_HUGE 2
 struct Grid {
-    unsigned M = 0;        // bucket count (power of two)
-    DVec<int> cnt;         // per item cell count, scanned in place (n+1)
+    unsigned M = 0;        // bucket count (power of two, >= 2 cap)
+    int cap = 0;           // entry capacity; grows when a build overflows
+    DVec<int> cnt;         // per item cell count, scanned in place (n+1); cnt[n] = entries
     DVec<unsigned> ekey;   // bucket of each entry
-    DVec<int> eitem;       // item of each entry
+    DVec<int> eitem;       // item | lowest-cell flags << 28
+    DVec<unsigned long long> ecell, cells;  // exact cell key per entry / ordered by bucket
     DVec<int> bstart;      // M+1 bucket offsets
     DVec<int> bfill;       // M cursors
     DVec<int> items;       // entries ordered by bucket
-    DVec<int> qoff;        // query chunk offsets (nq+1)
-    int nentries = 0;
+    DVec<int> qoff;        // query cell offsets (nq+1)
 };
 
 // HIP-event timing of single kernels: pairs are recorded on the context
@@ -108,18 +115,22 @@ struct fipc_ctx {
     size_t cub_bytes = 0;
     DVec<int> nuniq;
     // preconditioner + PCG
-    DVec<double> cdiag, Minv, r, z, pp, q;
+    DVec<double> cdiag, Minv, r, z, pp, q, qc;   // q = A_bsr p, qc = contact part of A p
     DVec<double> part_a, part_b, part_c, scal;
     DVec<int> pcg_state;   // [0] done_at, [1] iterations
     // reductions for energy / residual / ccd
     DVec<double> epart;
     DVec<unsigned long long> amin;
-    DVec<int> flag;
+    DVec<int> flag;        // [0] broad-phase status, [1] collision hit, [2] needed grid entries
+    // CCD candidate pairs (point-triangle, edge-edge) between the grid walk and the narrow phase
+    int ccd_cap = 0;
+    DVec<int2> ccd_pairs[2];
     // broad phase grids
     Grid grid;
     // statistics
     double last_pcg_relres = 0.0;
     int last_pcg_iters = 0;
+    int pcg_hint = 0;      // iterations of the last first round, sizes the first chunk
     KernelTimer tm[FIPC_NTIMERS];
     std::vector<hipEvent_t> event_pool;
     bool timing = false;

@@ -191,11 +191,12 @@ __global__ void __launch_bounds__(TPB) k_elastic(int nt, const int* __restrict__
 
 // second pass, one thread per BSR block: sum the contributing tet blocks in
 // a fixed order (transposed for the lower triangle), add the lumped mass on
-// diagonal blocks (compute_inertia, NEWTON_IPC.py:558-567), store the planes
-template <class T>
+// diagonal blocks (compute_inertia, NEWTON_IPC.py:558-567), store the FP64
+// planes and, when valf != nullptr, their FP32 copy for the FP32 SpMV
 __global__ void k_gather_blocks(int nnzb, const int* __restrict__ bcptr, const int* __restrict__ bcent,
                                 const double* __restrict__ tetH, const int* __restrict__ brow,
-                                const int* __restrict__ diag, const double* __restrict__ m, T* __restrict__ val)
+                                const int* __restrict__ diag, const double* __restrict__ m, double* __restrict__ val,
+                                float* __restrict__ valf)
 {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nnzb) return;
@@ -222,11 +223,13 @@ __global__ void k_gather_blocks(int nnzb, const int* __restrict__ bcptr, const i
         acc[0] += mi; acc[4] += mi; acc[8] += mi;
     }
 #pragma unroll
-    for (int k = 0; k < 9; k++) val[(size_t)k * nnzb + b] = (T)acc[k];
+    for (int k = 0; k < 9; k++) val[(size_t)k * nnzb + b] = acc[k];
+    if (valf) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) valf[(size_t)k * nnzb + b] = (float)acc[k];
+    }
 }
 
-// rhs = -m (x - xTilde) - sum of the incident tets' gradients (rhs -= g,
-// NEWTON_IPC.py:566, :679-690); clears the contact diagonal blocks
 __global__ void k_gather_rhs(int nv, const int* __restrict__ vcptr, const int* __restrict__ vcent,
                              const double* __restrict__ tetg, const double* __restrict__ m, const double* __restrict__ x,
                              const double* __restrict__ xt, const unsigned char* __restrict__ dmask, double* rhs,
@@ -366,15 +369,14 @@ __global__ void __launch_bounds__(TPB) k_contact(int n, const int* __restrict__ 
 
 // block-Jacobi preconditioner: inverse of (elastic diag block + contact diag
 // block); Dirichlet vertices get the identity (their residual is zero)
-template <class T>
-__global__ void k_precond(int nv, int nnzb, const T* __restrict__ val, const int* __restrict__ diag,
+__global__ void k_precond(int nv, int nnzb, const double* __restrict__ val, const int* __restrict__ diag,
                           const double* __restrict__ cdiag, const unsigned char* __restrict__ dmask, double* Minv)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nv) return;
     double A[9];
     int b = diag[i];
-    for (int k = 0; k < 9; k++) A[k] = (double)val[(size_t)k * nnzb + b] + cdiag[9 * i + k];
+    for (int k = 0; k < 9; k++) A[k] = val[(size_t)k * nnzb + b] + cdiag[9 * i + k];
     double* o = Minv + 9 * i;
     double d = det3(A);
     if (dmask[i] || !(fabs(d) > 0.0)) {
@@ -391,10 +393,19 @@ __global__ void k_precond(int nv, int nnzb, const T* __restrict__ val, const int
 // SpMV: q += A p  (BSR part) and q += sum_rows H_r p_r (contact part)
 // ---------------------------------------------------------------------------
 #define SPMV_G 8
-// 8 lanes per block row; lanes stride over the row's blocks, then reduce with
-// shuffles.  A block handles 32 consecutive rows per pass over a contiguous
-// range of row groups (xcd_block).  When part != 0 each block also stores its
-// partial of p.q (after Dirichlet masking) in part[blockIdx.x].
+// One launch computes A p: blocks [0, nb_c) run the contact rows of every
+// type (first, so that they overlap the BSR rows instead of trailing them),
+// the nb_bsr blocks after them the BSR rows (nb_c is a multiple of 8, so the
+// XCD of a BSR block is that of its own index).
+//  - BSR rows: 8 lanes per block row stride over the row's blocks and reduce
+//    with shuffles; a block handles 32 consecutive rows per pass over a
+//    contiguous range of row groups (xcd_block).  q[row] = (A_bsr p)[row] is
+//    stored (zero on Dirichlet rows), so q needs no clearing.
+//  - contact rows: a lane group per row, one lane per vertex; the lane's three
+//    rows of H_r p_r are added atomically into qc (skipping Dirichlet vertices).
+// With part != 0 every block stores its partial of p.(A p): p_row . q[row] for
+// BSR rows, p_v . (H_r p_r)_v for contact lanes (Dirichlet entries of the
+// PCG directions are zero, so this is the masked p.q).
 #define SPMV_ROWS (TPB / SPMV_G)
 #define SPMV_MAXB 1024
 static inline int spmv_grid(int nv)
@@ -403,65 +414,24 @@ static inline int spmv_grid(int nv)
     int nb = std::min(groups, SPMV_MAXB);
     return std::max(8, (nb + 7) & ~7);
 }
-template <class T>
-__global__ void __launch_bounds__(TPB) k_bsr_spmv(int nv, int nnzb, const int* __restrict__ rowptr,
-                                                  const int* __restrict__ col, const T* __restrict__ val,
-                                                  const double* __restrict__ p, double* __restrict__ q,
-                                                  const unsigned char* __restrict__ dmask, double* part,
-                                                  const int* __restrict__ state, int it)
-{
-    if (state && it > state[0]) return;
-    int nb = gridDim.x;
-    int lb = xcd_block(blockIdx.x, nb);
-    int groups = (nv + SPMV_ROWS - 1) / SPMV_ROWS;
-    int per = (groups + nb - 1) / nb;
-    int g0 = lb * per, g1 = min(g0 + per, groups);
-    int lane = threadIdx.x % SPMV_G;
-    double pq = 0.0;
-    for (int g = g0; g < g1; g++) {
-        int row = g * SPMV_ROWS + threadIdx.x / SPMV_G;
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        if (row < nv) {
-            int b0 = rowptr[row], b1 = rowptr[row + 1];
-            for (int b = b0 + lane; b < b1; b += SPMV_G) {
-                int c = col[b];
-                double x0 = p[3 * c], x1 = p[3 * c + 1], x2 = p[3 * c + 2];
-                a0 += (double)val[0 * nnzb + b] * x0 + (double)val[1 * nnzb + b] * x1 + (double)val[2 * nnzb + b] * x2;
-                a1 += (double)val[3 * nnzb + b] * x0 + (double)val[4 * nnzb + b] * x1 + (double)val[5 * nnzb + b] * x2;
-                a2 += (double)val[6 * nnzb + b] * x0 + (double)val[7 * nnzb + b] * x1 + (double)val[8 * nnzb + b] * x2;
-            }
-        }
-#pragma unroll
-        for (int o = SPMV_G / 2; o > 0; o >>= 1) {
-            a0 += __shfl_xor(a0, o, 64);
-            a1 += __shfl_xor(a1, o, 64);
-            a2 += __shfl_xor(a2, o, 64);
-        }
-        if (row < nv && lane == 0) {
-            double y0 = q[3 * row] + a0, y1 = q[3 * row + 1] + a1, y2 = q[3 * row + 2] + a2;
-            if (dmask[row]) y0 = y1 = y2 = 0.0;
-            q[3 * row] = y0; q[3 * row + 1] = y1; q[3 * row + 2] = y2;
-            pq += p[3 * row] * y0 + p[3 * row + 1] * y1 + p[3 * row + 2] * y2;
-        }
-    }
-    if (part) {
-        pq = block_sum(pq);
-        if (threadIdx.x == 0) part[blockIdx.x] = pq;
-    }
-}
-
+// contact rows of the seven types, and the first block of each type counted
+// from the first contact block
+struct ContactSet {
+    const int* rows[NTYPES];
+    const double* H[NTYPES];
+    int n[NTYPES];
+    int boff[NTYPES + 1];
+};
 template <int KIND>
-__global__ void k_contact_spmv(int n, const int* __restrict__ rows, const double* __restrict__ Hc,
-                               const double* __restrict__ p, double* q, const unsigned char* __restrict__ dmask,
-                               const int* __restrict__ state, int it)
+__device__ __forceinline__ double contact_rows_spmv(int n, const int* __restrict__ rows, const double* __restrict__ Hc,
+                                                    const double* __restrict__ p, double* qc,
+                                                    const unsigned char* __restrict__ dmask, int gid)
 {
-    if (state && it > state[0]) return;
     using KI = KindInfo<KIND>;
     constexpr int N = KI::N, K = KI::K;
     constexpr int W = KIND == 0 ? 2 : (KIND == 1 ? 3 : 4);
-    int gid = blockIdx.x * blockDim.x + threadIdx.x;
     int r = gid / K, lane = gid % K;
-    if (r >= n) return;
+    if (r >= n) return 0.0;
     double pv[N];
 #pragma unroll
     for (int a = 0; a < K; a++) {
@@ -470,13 +440,80 @@ __global__ void k_contact_spmv(int n, const int* __restrict__ rows, const double
     }
     const double* Hr = Hc + (size_t)r * N * N + lane * 3 * N;
     int v = rows[W * r + lane];
-    if (dmask[v]) return;  // Dirichlet rows of q are zeroed afterwards
+    if (dmask[v]) return 0.0;
+    double pq = 0.0;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         double s = 0.0;
 #pragma unroll
         for (int c = 0; c < N; c++) s += Hr[i * N + c] * pv[c];
-        atomicAdd(&q[3 * v + i], s);
+        atomicAdd(&qc[3 * v + i], s);
+        pq += p[3 * v + i] * s;
+    }
+    return pq;
+}
+template <class T>
+__global__ void __launch_bounds__(TPB) k_spmv(int nv, int nnzb, const int* __restrict__ rowptr,
+                                              const int* __restrict__ col, const T* __restrict__ val,
+                                              const double* __restrict__ p, double* __restrict__ q, double* qc,
+                                              const unsigned char* __restrict__ dmask, ContactSet cs, int nb_c,
+                                              int nb_bsr, double* part, const int* __restrict__ state, int it)
+{
+    if (state && it > state[0]) return;
+    double pq = 0.0;
+    if ((int)blockIdx.x >= nb_c) {
+        int lb = xcd_block((int)blockIdx.x - nb_c, nb_bsr);
+        int groups = (nv + SPMV_ROWS - 1) / SPMV_ROWS;
+        int per = (groups + nb_bsr - 1) / nb_bsr;
+        int g0 = lb * per, g1 = min(g0 + per, groups);
+        int lane = threadIdx.x % SPMV_G;
+        int rsub = threadIdx.x / SPMV_G;
+        // row bounds of the next group are loaded while the current one runs
+        int row = g0 * SPMV_ROWS + rsub;
+        int nb0 = row < nv && g0 < g1 ? rowptr[row] : 0, nb1 = row < nv && g0 < g1 ? rowptr[row + 1] : 0;
+        for (int g = g0; g < g1; g++) {
+            row = g * SPMV_ROWS + rsub;
+            int b0 = nb0, b1 = nb1;
+            int nrow = row + SPMV_ROWS;
+            if (g + 1 < g1 && nrow < nv) { nb0 = rowptr[nrow]; nb1 = rowptr[nrow + 1]; }
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+            for (int b = b0 + lane; b < b1; b += SPMV_G) {
+                int c = col[b];
+                double x0 = p[3 * c], x1 = p[3 * c + 1], x2 = p[3 * c + 2];
+                a0 += (double)val[0 * nnzb + b] * x0 + (double)val[1 * nnzb + b] * x1 + (double)val[2 * nnzb + b] * x2;
+                a1 += (double)val[3 * nnzb + b] * x0 + (double)val[4 * nnzb + b] * x1 + (double)val[5 * nnzb + b] * x2;
+                a2 += (double)val[6 * nnzb + b] * x0 + (double)val[7 * nnzb + b] * x1 + (double)val[8 * nnzb + b] * x2;
+            }
+#pragma unroll
+            for (int o = SPMV_G / 2; o > 0; o >>= 1) {
+                a0 += __shfl_xor(a0, o, 64);
+                a1 += __shfl_xor(a1, o, 64);
+                a2 += __shfl_xor(a2, o, 64);
+            }
+            if (row < nv && lane == 0) {
+                if (dmask[row]) a0 = a1 = a2 = 0.0;
+                q[3 * row] = a0; q[3 * row + 1] = a1; q[3 * row + 2] = a2;
+                pq += p[3 * row] * a0 + p[3 * row + 1] * a1 + p[3 * row + 2] * a2;
+            }
+        }
+    } else {
+        int cb = (int)blockIdx.x;
+        int t = 0;
+        while (t < NTYPES - 1 && cb >= cs.boff[t + 1]) t++;
+        int gid = (cb - cs.boff[t]) * TPB + (int)threadIdx.x;
+        switch (t) {
+        case 0: pq = contact_rows_spmv<0>(cs.n[0], cs.rows[0], cs.H[0], p, qc, dmask, gid); break;
+        case 1: pq = contact_rows_spmv<1>(cs.n[1], cs.rows[1], cs.H[1], p, qc, dmask, gid); break;
+        case 2: pq = contact_rows_spmv<2>(cs.n[2], cs.rows[2], cs.H[2], p, qc, dmask, gid); break;
+        case 3: pq = contact_rows_spmv<3>(cs.n[3], cs.rows[3], cs.H[3], p, qc, dmask, gid); break;
+        case 4: pq = contact_rows_spmv<4>(cs.n[4], cs.rows[4], cs.H[4], p, qc, dmask, gid); break;
+        case 5: pq = contact_rows_spmv<5>(cs.n[5], cs.rows[5], cs.H[5], p, qc, dmask, gid); break;
+        default: pq = contact_rows_spmv<6>(cs.n[6], cs.rows[6], cs.H[6], p, qc, dmask, gid); break;
+        }
+    }
+    if (part) {
+        pq = block_sum(pq);
+        if (threadIdx.x == 0) part[blockIdx.x] = pq;
     }
 }
 
@@ -486,29 +523,30 @@ __global__ void k_contact_spmv(int n, const int* __restrict__ rows, const double
 // initial guess and a zero residual afterwards.
 // ---------------------------------------------------------------------------
 // x = p0 (prescribed on Dirichlet dofs, 0 elsewhere)
-__global__ void k_pcg_x0(int nv, const double* xt, const double* xcur, const unsigned char* dmask, double* sol, double* q)
+__global__ void k_pcg_x0(int nv, const double* xt, const double* xcur, const unsigned char* dmask, double* sol, double* qc)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nv) return;
     for (int d = 0; d < 3; d++) {
         sol[3 * i + d] = dmask[i] ? xt[3 * i + d] - xcur[3 * i + d] : 0.0;
-        q[3 * i + d] = 0.0;
+        qc[3 * i + d] = 0.0;
     }
 }
-// r = b - q (q = A x0, zero when x0 = 0); z = Minv r; p = z; partials of r.z and b.b
-__global__ void k_pcg_init(int nv, const double* b, const double* q, const unsigned char* dmask, const double* Minv,
-                           double* r, double* z, double* p, double* qz, double* part_rz, double* part_bb)
+// r = b - (q + qc) (= b - A x); z = Minv r; p = z; qc cleared for the
+// iterations; partials of r.z and r.r
+__global__ void k_pcg_init(int nv, const double* b, const double* q, double* qc, const unsigned char* dmask,
+                           const double* Minv, double* r, double* z, double* p, double* part_rz, double* part_bb)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0, bb = 0.0;
     if (i < nv) {
         double ri[3], zi[3];
         bool fixed = dmask[i];
-        for (int d = 0; d < 3; d++) ri[d] = fixed ? 0.0 : b[3 * i + d] - q[3 * i + d];
+        for (int d = 0; d < 3; d++) ri[d] = fixed ? 0.0 : b[3 * i + d] - (q[3 * i + d] + qc[3 * i + d]);
         const double* M = Minv + 9 * i;
         for (int d = 0; d < 3; d++) zi[d] = fixed ? 0.0 : M[3 * d] * ri[0] + M[3 * d + 1] * ri[1] + M[3 * d + 2] * ri[2];
         for (int d = 0; d < 3; d++) {
-            r[3 * i + d] = ri[d]; z[3 * i + d] = zi[d]; p[3 * i + d] = zi[d]; qz[3 * i + d] = 0.0;
+            r[3 * i + d] = ri[d]; z[3 * i + d] = zi[d]; p[3 * i + d] = zi[d]; qc[3 * i + d] = 0.0;
             rz += ri[d] * zi[d];
             bb += ri[d] * ri[d];
         }
@@ -517,14 +555,24 @@ __global__ void k_pcg_init(int nv, const double* b, const double* q, const unsig
     bb = block_sum(bb);
     if (threadIdx.x == 0) { part_rz[blockIdx.x] = rz; part_bb[blockIdx.x] = bb; }
 }
-// scal[0] = rz, scal[2] = bb from partials
-__global__ void k_pcg_scalars(const double* part_rz, const double* part_bb, int n, double* scal, int* state)
+// scal[0|1] = rz from partials.  First start: scal[2|3] = |r0|^2, the
+// reference norm of the relative tolerance.  Restart (refinement): the
+// partials hold |r|^2 of the recomputed FP64 residual, kept in scal[4];
+// state[0] = -1 when that residual already meets the tolerance.
+__global__ void k_pcg_scalars(const double* part_rz, const double* part_bb, int n, double* scal, int* state,
+                              int restart, double tol2)
 {
     double rz = reduce_partials(part_rz, n);
     double bb = reduce_partials(part_bb, n);
     if (threadIdx.x == 0) {
-        scal[0] = rz; scal[1] = rz; scal[2] = bb; scal[3] = bb;
-        state[0] = bb == 0.0 ? -1 : 0x7fffffff;  // done_at
+        scal[0] = rz; scal[1] = rz;
+        if (!restart) {
+            scal[2] = bb; scal[3] = bb; scal[4] = bb;
+            state[0] = bb == 0.0 ? -1 : 0x7fffffff;  // done_at
+        } else {
+            scal[4] = bb;
+            state[0] = (bb <= tol2 * scal[2] || !(bb == bb)) ? -1 : 0x7fffffff;
+        }
         state[1] = 0;
     }
 }
@@ -539,7 +587,8 @@ static inline int pcg_grid(int nv) { return std::min(nblk(nv), PCG_MAXB); }
 // z = Minv r; partials of r.z and r.r
 __global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const unsigned char* __restrict__ dmask,
                                                   const double* __restrict__ Minv, const double* __restrict__ p,
-                                                  const double* __restrict__ q, double* x, double* r, double* z,
+                                                  const double* __restrict__ q, const double* __restrict__ qc,
+                                                  double* x, double* r, double* z,
                                                   const double* part_pq, int npq, double* part_rz, double* part_rr,
                                                   const int* state, int it)
 {
@@ -553,7 +602,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const un
         double ri[3];
         for (int d = 0; d < 3; d++) {
             x[3 * i + d] += alpha * p[3 * i + d];
-            ri[d] = r[3 * i + d] - alpha * q[3 * i + d];
+            ri[d] = r[3 * i + d] - alpha * (q[3 * i + d] + qc[3 * i + d]);
             r[3 * i + d] = ri[d];
         }
         const double* M = Minv + 9 * i;
@@ -572,9 +621,9 @@ __global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const un
     }
 }
 // reduces r.z and r.r, runs the convergence test (block 0 records it),
-// beta = rz_new / rz; p = z + beta p; q = 0 for the next product
+// beta = rz_new / rz; p = z + beta p; qc = 0 for the next product
 __global__ void __launch_bounds__(TPB) k_pcg_upd2(int nv, double* scal, const double* __restrict__ z, double* p,
-                                                  double* q, const double* part_rz, const double* part_rr, int n1,
+                                                  double* qc, const double* part_rz, const double* part_rr, int n1,
                                                   int* state, int it, double tol2)
 {
     if (it > state[0]) return;
@@ -589,7 +638,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_upd2(int nv, double* scal, const do
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nv; i += gridDim.x * blockDim.x) {
         p[i] = z[i] + beta * p[i];
-        q[i] = 0.0;
+        qc[i] = 0.0;
     }
 }
 
@@ -768,11 +817,12 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->vcent);
     for (int t = 0; t < NTYPES; t++) { release(c->rows[t]); release(c->cH[t]); }
     release(c->keys_pp); release(c->keys_pe); release(c->keys_tmp); release(c->counters); release(c->cub_tmp);
-    release(c->nuniq); release(c->cdiag); release(c->Minv); release(c->r); release(c->z); release(c->pp); release(c->q);
+    release(c->nuniq); release(c->cdiag); release(c->Minv); release(c->r); release(c->z); release(c->pp); release(c->q); release(c->qc);
     release(c->part_a); release(c->part_b); release(c->part_c); release(c->scal); release(c->pcg_state);
     release(c->epart); release(c->amin); release(c->flag);
     release(c->grid.cnt); release(c->grid.ekey); release(c->grid.eitem); release(c->grid.bstart);
-    release(c->grid.bfill); release(c->grid.items);
+    release(c->grid.bfill); release(c->grid.items); release(c->grid.ecell); release(c->grid.cells);
+    release(c->grid.qoff); release(c->ccd_pairs[0]); release(c->ccd_pairs[1]);
     (void)fipc_tm_fold(c);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->s);
@@ -876,7 +926,7 @@ extern "C" int fipc_upload_mesh(fipc_ctx* c, int nv, const double* x0, const dou
     size_t n3 = (size_t)nv * 3;
     STCHK(ensure(c->x, n3)); STCHK(ensure(c->xn, n3)); STCHK(ensure(c->xt, n3)); STCHK(ensure(c->xprev, n3));
     STCHK(ensure(c->v, n3)); STCHK(ensure(c->sol, n3)); STCHK(ensure(c->rhs, n3)); STCHK(ensure(c->target, n3));
-    STCHK(ensure(c->r, n3)); STCHK(ensure(c->z, n3)); STCHK(ensure(c->pp, n3)); STCHK(ensure(c->q, n3));
+    STCHK(ensure(c->r, n3)); STCHK(ensure(c->z, n3)); STCHK(ensure(c->pp, n3)); STCHK(ensure(c->q, n3)); STCHK(ensure(c->qc, n3));
     STCHK(ensure(c->cdiag, (size_t)nv * 9)); STCHK(ensure(c->Minv, (size_t)nv * 9));
     HIPCHK(hipMemcpyAsync(c->x.p, x0, n3 * 8, hipMemcpyHostToDevice, c->s));
     HIPCHK(hipMemcpyAsync(c->xt.p, x0, n3 * 8, hipMemcpyHostToDevice, c->s));
@@ -891,6 +941,9 @@ extern "C" int fipc_upload_mesh(fipc_ctx* c, int nv, const double* x0, const dou
     STCHK(ensure(c->scal, 8)); STCHK(ensure(c->pcg_state, 4));
     STCHK(ensure(c->epart, nblk(nmax) * 4 + 64));
     STCHK(ensure(c->amin, 4)); STCHK(ensure(c->flag, 4)); STCHK(ensure(c->counters, 16)); STCHK(ensure(c->nuniq, 4));
+    // broad-phase capacities start from the primitive counts and grow on demand
+    if (c->grid.cap == 0) c->grid.cap = 4 * std::max(nbp, std::max(nbe, nbt)) + 4096;
+    if (c->ccd_cap == 0) c->ccd_cap = 65536;
     for (int t = 0; t < NTYPES; t++) {
         if (c->cap[t] == 0) c->cap[t] = 1024;
         int w = t == 0 ? 2 : (t == 1 ? 3 : 4);
@@ -1039,14 +1092,9 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_ELASTIC));
     STCHK(fipc_tm_begin(c, TM_GATHER));
-    if (precision == 1) {
-        STCHK(ensure(c->valf, (size_t)c->nnzb * 9));
-        k_gather_blocks<float><<<nblk(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->bcent.p, c->tetH.p, c->brow.p,
-                                                                 c->diag.p, c->m.p, c->valf.p);
-    } else {
-        k_gather_blocks<double><<<nblk(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->bcent.p, c->tetH.p, c->brow.p,
-                                                                  c->diag.p, c->m.p, c->val.p);
-    }
+    if (precision == 1) STCHK(ensure(c->valf, (size_t)c->nnzb * 9));
+    k_gather_blocks<<<nblk(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->bcent.p, c->tetH.p, c->brow.p, c->diag.p,
+                                                     c->m.p, c->val.p, precision == 1 ? c->valf.p : nullptr);
     k_gather_rhs<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->vcptr.p, c->vcent.p, c->tetg.p, c->m.p, c->x.p, c->xt.p,
                                                  c->dmask.p, c->rhs.p, c->cdiag.p);
     HIPCHK(hipGetLastError());
@@ -1054,105 +1102,155 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     STCHK(fipc_tm_begin(c, TM_CONTACT));
     STCHK(launch_contacts(c, 1));
     STCHK(fipc_tm_end(c, TM_CONTACT));
-    if (precision == 1)
-        k_precond<float><<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->nnzb, c->valf.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p);
-    else
-        k_precond<double><<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->nnzb, c->val.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p);
+    k_precond<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->nnzb, c->val.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p);
     HIPCHK(hipGetLastError());
     return FIPC_OK;
 }
 
 // ---- SpMV ----
-static int spmv(fipc_ctx* c, const double* p, double* q, double* part, const int* state, int it)
+// contact blocks follow the BSR blocks; returns the total grid size
+static int spmv_contacts(fipc_ctx* c, ContactSet& cs)
 {
+    int nb = 0;
     for (int t = 0; t < NTYPES; t++) {
-        int n = c->count[t];
-        if (!n) continue;
         int K = t == 0 ? 2 : (t == 1 ? 3 : 4);
-        int nb = nblk((long)n * K);
-#define LAUNCH_S(KK) k_contact_spmv<KK><<<nb, TPB, 0, c->s>>>(n, c->rows[t].p, c->cH[t].p, p, q, c->dmask.p, state, it)
-        switch (t) {
-        case 0: LAUNCH_S(0); break;
-        case 1: LAUNCH_S(1); break;
-        case 2: LAUNCH_S(2); break;
-        case 3: LAUNCH_S(3); break;
-        case 4: LAUNCH_S(4); break;
-        case 5: LAUNCH_S(5); break;
-        default: LAUNCH_S(6); break;
-        }
-#undef LAUNCH_S
-        HIPCHK(hipGetLastError());
+        cs.rows[t] = c->rows[t].p;
+        cs.H[t] = c->cH[t].p;
+        cs.n[t] = c->count[t];
+        cs.boff[t] = nb;
+        nb += nblk((long)c->count[t] * K);
     }
-    int nb = spmv_grid(c->nv);
+    cs.boff[NTYPES] = nb;
+    return (nb + 7) & ~7;
+}
+// q + qc = A p (q = BSR part, qc = contact part accumulated: qc must be zero
+// on entry); prec selects the FP32 copy of the BSR values
+static int spmv(fipc_ctx* c, int prec, const double* p, double* q, double* qc, double* part, const int* state, int it)
+{
+    ContactSet cs;
+    int nbc = spmv_contacts(c, cs);
+    int nbb = spmv_grid(c->nv);
     STCHK(fipc_tm_begin(c, TM_SPMV));
-    if (c->precision == 1)
-        k_bsr_spmv<float><<<nb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->valf.p, p, q, c->dmask.p, part,
-                                                 state, it);
+    if (prec == 1)
+        k_spmv<float><<<nbc + nbb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->valf.p, p, q, qc, c->dmask.p,
+                                                   cs, nbc, nbb, part, state, it);
     else
-        k_bsr_spmv<double><<<nb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->val.p, p, q, c->dmask.p, part,
-                                                  state, it);
+        k_spmv<double><<<nbc + nbb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->val.p, p, q, qc, c->dmask.p,
+                                                    cs, nbc, nbb, part, state, it);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_SPMV));
     return FIPC_OK;
+}
+// partials of one product (contact blocks, padded to a multiple of 8, then BSR blocks)
+static int spmv_nparts(fipc_ctx* c)
+{
+    ContactSet cs;
+    return spmv_grid(c->nv) + spmv_contacts(c, cs);
+}
+__global__ void k_add(int n, const double* a, double* b)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] += a[i];
 }
 extern "C" int fipc_spmv(fipc_ctx* c, const double* x, double* y)
 {
     if (!c || !c->nv) return FIPC_ERR_STATE;
     size_t n = (size_t)c->nv * 24;
     HIPCHK(hipMemcpyAsync(c->pp.p, x, n, hipMemcpyHostToDevice, c->s));
-    HIPCHK(hipMemsetAsync(c->q.p, 0, n, c->s));
-    STCHK(spmv(c, c->pp.p, c->q.p, nullptr, nullptr, 0));
+    HIPCHK(hipMemsetAsync(c->qc.p, 0, n, c->s));
+    STCHK(spmv(c, 0, c->pp.p, c->q.p, c->qc.p, nullptr, nullptr, 0));
+    k_add<<<nblk((long)c->nv * 3), TPB, 0, c->s>>>(c->nv * 3, c->qc.p, c->q.p);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(y, c->q.p, n, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     return FIPC_OK;
 }
 
+// solve_system (NEWTON_IPC.py:750-777): block-Jacobi PCG from the prescribed
+// Dirichlet increments.  With the FP32 Hessian the iterations use the FP32
+// copy of the elastic blocks and are restarted from the residual b - A x
+// recomputed with the FP64 blocks until that residual meets the tolerance
+// (iterative refinement); with the FP64 Hessian there is a single round.
 extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double* relres)
 {
     if (!c || !c->nv) return FIPC_ERR_STATE;
     int nv = c->nv;
     int nbv = nblk(nv);
-    // x0 = prescribed Dirichlet increments; q = A x0
-    k_pcg_x0<<<nbv, TPB, 0, c->s>>>(nv, c->xt.p, c->x.p, c->dmask.p, c->sol.p, c->q.p);
-    HIPCHK(hipGetLastError());
-    STCHK(spmv(c, c->sol.p, c->q.p, nullptr, nullptr, 0));
-    k_pcg_init<<<nbv, TPB, 0, c->s>>>(nv, c->rhs.p, c->q.p, c->dmask.p, c->Minv.p, c->r.p, c->z.p, c->pp.p, c->q.p,
-                                       c->part_a.p, c->part_b.p);
-    HIPCHK(hipGetLastError());
-    k_pcg_scalars<<<1, TPB, 0, c->s>>>(c->part_a.p, c->part_b.p, nbv, c->scal.p, c->pcg_state.p);
-    HIPCHK(hipGetLastError());
-    int it = 0, state[2] = {0, 0};
-    const int chunk = 16;
-    const int nsp = spmv_grid(nv), nu = pcg_grid(nv);
-    double tol2 = tol * tol;
-    while (it < maxit) {
-        int end = std::min(maxit, it + chunk);
-        for (; it < end; it++) {
-            STCHK(spmv(c, c->pp.p, c->q.p, c->part_c.p, c->pcg_state.p, it));
-            STCHK(fipc_tm_begin(c, TM_PCGVEC));
-            k_pcg_upd1<<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->dmask.p, c->Minv.p, c->pp.p, c->q.p, c->sol.p, c->r.p,
-                                              c->z.p, c->part_c.p, nsp, c->part_a.p, c->part_b.p, c->pcg_state.p, it);
-            k_pcg_upd2<<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->z.p, c->pp.p, c->q.p, c->part_a.p, c->part_b.p, nu,
-                                              c->pcg_state.p, it, tol2);
-            STCHK(fipc_tm_end(c, TM_PCGVEC));
+    const int chunk = 8;
+    const int nsp = spmv_nparts(c), nu = pcg_grid(nv);
+    const double tol2 = tol * tol;
+    const int prec = c->precision;
+    STCHK(ensure(c->part_c, (size_t)nsp + 64));
+    const int rounds = prec == 1 ? 4 : 1;
+    int total = 0, state[2] = {0, 0};
+    bool converged = false;
+    for (int round = 0; round < rounds; round++) {
+        if (round == 0) {
+            // x0 = prescribed Dirichlet increments
+            k_pcg_x0<<<nbv, TPB, 0, c->s>>>(nv, c->xt.p, c->x.p, c->dmask.p, c->sol.p, c->qc.p);
+            HIPCHK(hipGetLastError());
+        } else {
+            HIPCHK(hipMemsetAsync(c->qc.p, 0, (size_t)nv * 24, c->s));
         }
+        // q + qc = A x with the FP64 blocks; r = b - A x, z = Minv r, p = z
+        STCHK(spmv(c, 0, c->sol.p, c->q.p, c->qc.p, nullptr, nullptr, 0));
+        k_pcg_init<<<nbv, TPB, 0, c->s>>>(nv, c->rhs.p, c->q.p, c->qc.p, c->dmask.p, c->Minv.p, c->r.p, c->z.p, c->pp.p,
+                                           c->part_a.p, c->part_b.p);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
-        if (state[0] != 0x7fffffff) break;
+        k_pcg_scalars<<<1, TPB, 0, c->s>>>(c->part_a.p, c->part_b.p, nbv, c->scal.p, c->pcg_state.p, round > 0, tol2);
+        HIPCHK(hipGetLastError());
+        int it = 0, budget = maxit - total;
+        state[0] = 0x7fffffff;
+        if (round > 0) {
+            // the recomputed residual may already meet the tolerance
+            HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
+            HIPCHK(hipStreamSynchronize(c->s));
+            if (state[0] == -1) { converged = true; break; }
+        }
+        // the host checks convergence between chunks of iterations; the first
+        // chunk is sized from the previous solve, whose count is usually close
+        int first = round == 0 ? std::max(chunk, c->pcg_hint - chunk / 2) : chunk;
+        while (it < budget) {
+            int end = std::min(budget, it + (it == 0 ? first : chunk));
+            for (; it < end; it++) {
+                STCHK(spmv(c, prec, c->pp.p, c->q.p, c->qc.p, c->part_c.p, c->pcg_state.p, it));
+                STCHK(fipc_tm_begin(c, TM_PCGVEC));
+                k_pcg_upd1<<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->dmask.p, c->Minv.p, c->pp.p, c->q.p, c->qc.p, c->sol.p,
+                                                  c->r.p, c->z.p, c->part_c.p, nsp, c->part_a.p, c->part_b.p,
+                                                  c->pcg_state.p, it);
+                k_pcg_upd2<<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->z.p, c->pp.p, c->qc.p, c->part_a.p, c->part_b.p, nu,
+                                                  c->pcg_state.p, it, tol2);
+                STCHK(fipc_tm_end(c, TM_PCGVEC));
+            }
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
+            HIPCHK(hipStreamSynchronize(c->s));
+            if (state[0] != 0x7fffffff) break;
+        }
+        if (budget <= 0) {
+            HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
+            HIPCHK(hipStreamSynchronize(c->s));
+        }
+        int ran = state[0] == -1 ? 0 : state[1];
+        total += ran;
+        if (round == 0 && ran > 0) c->pcg_hint = ran;
+        if (state[0] == 0x7fffffff) break;        // iteration budget spent
+        if (prec != 1 || state[0] == -1) { converged = true; break; }
+        // FP32 iterations converged: the next round checks the FP64 residual
     }
+    // refinement rounds exhausted after a converged FP32 round: accept it
+    // (relres then reports the recursive residual of that round)
+    if (!converged && state[0] != 0x7fffffff) converged = true;
     double sc[5];
     HIPCHK(hipMemcpyAsync(sc, c->scal.p, 40, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
-    int n_it = state[0] == -1 ? 0 : state[1];
     double rel = sc[2] > 0 ? sqrt(sc[4] / sc[2]) : 0.0;
-    if (state[0] == -1) rel = 0.0;
-    c->last_pcg_iters = n_it;
+    c->last_pcg_iters = total;
     c->last_pcg_relres = rel;
-    if (iters) *iters = n_it;
+    if (iters) *iters = total;
     if (relres) *relres = rel;
     if (!(rel == rel)) return FIPC_ERR_NONFINITE;
-    if (state[0] == 0x7fffffff) return FIPC_ERR_NOT_CONVERGED;
+    if (!converged) return FIPC_ERR_NOT_CONVERGED;
     return FIPC_OK;
 }
 
@@ -1240,6 +1338,39 @@ int fipc_inversion_alpha(fipc_ctx* c, unsigned long long* dst)
     if (!c->nt) return FIPC_OK;
     k_inversion<<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->tets.p, c->x.p, c->sol.p, dst);
     HIPCHK(hipGetLastError());
+    return FIPC_OK;
+}
+
+// plain streaming copy between two device buffers (bandwidth reference and
+// hardware-counter calibration: its algorithmic traffic is exactly 2 n bytes)
+__global__ void k_stream_copy(long n2, const double2* __restrict__ a, double2* __restrict__ b)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+}
+extern "C" int fipc_stream_copy(fipc_ctx* c, long nbytes, double* ms)
+{
+    if (!c || nbytes <= 0 || nbytes % 16) return FIPC_ERR_ARG;
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, nbytes));
+    HIPCHK(hipMalloc(&b, nbytes));
+    HIPCHK(hipMemsetAsync(a, 0, nbytes, c->s));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    long n2 = nbytes / 16;
+    int nb = (int)std::min<long>((n2 + TPB - 1) / TPB, 8192);
+    HIPCHK(hipEventRecord(e0, c->s));
+    k_stream_copy<<<nb, TPB, 0, c->s>>>(n2, (const double2*)a, (double2*)b);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, c->s));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
     return FIPC_OK;
 }
 

@@ -172,6 +172,11 @@ int fipc_spmv(fipc_ctx* c, const double* x, double* y);
 int fipc_enable_timing(fipc_ctx* c, int on);
 int fipc_kernel_times(fipc_ctx* c, int n, double* ms, long* launches);
 int fipc_sync(fipc_ctx* c);
+/* plain device-to-device copy of nbytes (multiple of 16) between two
+ * temporary buffers on the context stream; *ms = its HIP-event time.  A
+ * bandwidth reference and the calibration kernel of hardware byte counters
+ * (no counterpart in the reference). */
+int fipc_stream_copy(fipc_ctx* c, long nbytes, double* ms);
 
 #ifdef __cplusplus
 }
