@@ -11,9 +11,11 @@ each ending with a device synchronisation.  The position state is resident in
 HBM for the whole run.
 
 Printed: one JSON line with steps/s, the roofline figures of the dominant
-kernel (HIP-event durations of its launches during one extra step, against
-its algorithmic bytes) and a CPU baseline (the C oracle, OpenMP, with a
-direct sparse solve) on a bounded sample of the same scene type.
+kernel (HIP-event durations of its launches over the timed steps, against its
+algorithmic bytes; the HBM traffic per launch measured with rocprofv3
+FETCH_SIZE / WRITE_SIZE, read from profiles/k_spmv_traffic_compressed_slab.md)
+and a CPU baseline (the C oracle, OpenMP, with a direct sparse solve) on a
+bounded sample of the same scene type.
 """
 from __future__ import annotations
 
@@ -31,11 +33,26 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TIMER_NAMES = ("spmv", "elastic_tet_pass", "ccd_query", "contact_query", "energy", "contact_assembly",
-               "pcg_vector", "grid_build", "assembly_gather")
+               "pcg_vector", "grid_build", "assembly_gather", "spmv_residual")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic(mode):
+    """HBM bytes per SpMV launch from the committed rocprofv3 counter summary
+    (a line 'traffic_bytes_per_launch: plain=N fp32=M'), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_spmv_traffic_compressed_slab.md")
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*plain=(\d+)\s+fp32=(\d+)", txt)
+    if not m:
+        return None
+    return int(m.group(1) if mode == "plain" else m.group(2))
 
 
 def algorithmic_bytes(sim, precision):
@@ -76,6 +93,10 @@ def run_gpu(args):
         log(f"warmup step {i}: {time.perf_counter() - t1:.3f}s newton {st.newton_iters} pcg {st.pcg_iters} "
             f"contacts {st.counts[-1] if st.counts else None}")
     _lib.call("fipc_sync", sim.ctx)
+    # HIP-event timing of the kernel classes over the timed region (events are
+    # recorded on the context stream without synchronising)
+    if not args.traffic_probe:
+        _lib.call("fipc_enable_timing", sim.ctx, 1)
     t0 = time.perf_counter()
     pcg = []
     for i in range(args.steps):
@@ -89,16 +110,13 @@ def run_gpu(args):
     _lib.call("fipc_sync", sim.ctx)
     wall = time.perf_counter() - t0
     if args.traffic_probe:
-        # counter-calibration run: a plain device copy of known size, no event timing
+        # counter-calibration run: a plain device copy of known size
         cms = C.c_double()
         _lib.call("fipc_stream_copy", sim.ctx, 256 << 20, C.byref(cms))
         log(f"traffic probe: copied {256 << 20} bytes device to device in {cms.value:.3f} ms "
             f"({2 * (256 << 20) / cms.value / 1e6:.0f} GB/s)")
         sim.close()
         return None
-    # one extra step with kernel timing on
-    _lib.call("fipc_enable_timing", sim.ctx, 1)
-    st = sim.step()
     nt_ = len(TIMER_NAMES)
     ms = np.zeros(nt_)
     la = (C.c_long * nt_)()
@@ -117,8 +135,9 @@ def run_gpu(args):
     avg_s = kt[dom]["us_avg"] * 1e-6
     achieved = nbytes[dom] / avg_s / 1e9 if avg_s > 0 else 0.0
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": nbytes[dom],
-            "avg_launch_us": round(kt[dom]["us_avg"], 2)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(args.mode) if dom == "spmv" else None, "bytes_per_launch": nbytes[dom],
+            "avg_launch_us": round(kt[dom]["us_avg"], 2), "launches": kt[dom]["launches"]}
     others = {k: {"achieved_GBs": round(nbytes[k] / (kt[k]["us_avg"] * 1e-6) / 1e9, 1) if kt[k]["us_avg"] else None,
                   "avg_launch_us": round(kt[k]["us_avg"], 2)} for k in nbytes}
     res = {
@@ -129,7 +148,7 @@ def run_gpu(args):
         "contacts_last_step": {n: int(c) for n, c in zip(("PP", "PE", "PT", "EE", "EEM", "PPM", "PEM"), counts)},
         "roofline": roof,
         "kernel_classes": others,
-        "kernel_times_one_step": kt,
+        "kernel_times_timed_region": kt,
         "setup_s": setup_s,
         "nv": sim.nv,
         "nt": len(sc.tets),

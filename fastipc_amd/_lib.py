@@ -20,6 +20,9 @@ _D = C.c_double
 _P = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int)
 _VP = C.c_void_p
+_FP = C.POINTER(C.c_float)
+_LP = C.POINTER(C.c_long)
+_U64 = C.c_ulonglong
 
 STATUS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "capacity exceeded", 4: "non-finite value",
           5: "PCG not converged", 6: "call out of order"}
@@ -65,6 +68,22 @@ _SIGS = {
     "fipc_batch_ccd": [C.c_int, C.c_int, _P, _P, _D, _P, _P],
     "fipc_host_constraint": [C.c_int, _P, _P, _D, _D, C.c_int, _P, _P, _P],
     "fipc_host_tet": [_P, _P, _D, _D, _D, _P, _P, _P],
+    "fipc_mlp_create": [C.POINTER(_VP), C.c_int, C.c_int, C.c_int, C.c_int, _U64],
+    "fipc_mlp_destroy": [_VP],
+    "fipc_mlp_nparams": [_VP, _LP],
+    "fipc_mlp_get_params": [_VP, _FP],
+    "fipc_mlp_set_params": [_VP, _FP],
+    "fipc_mlp_forward": [_VP, C.c_int, _FP, _FP],
+    "fipc_mlp_grad": [_VP, C.c_int, _FP, _FP, _FP, _FP],
+    "fipc_mlp_dataset_clear": [_VP],
+    "fipc_mlp_dataset_size": [_VP, _LP],
+    "fipc_mlp_dataset_add": [_VP, C.c_int, _FP, _FP],
+    "fipc_mlp_dataset_get": [_VP, C.c_long, C.c_long, _FP, _FP],
+    "fipc_mlp_train": [_VP, C.c_int, C.c_int, C.c_float, _U64, _FP],
+    "fipc_ws_features": [_VP, _D, _FP],
+    "fipc_ws_predict": [_VP, _VP, _D],
+    "fipc_ws_capture_begin": [_VP, _VP, _D, C.c_int, _U64],
+    "fipc_ws_capture_end": [_VP, _VP, _D],
 }
 _SCALAR = {
     "project_pd_3": [_D] * 9 + [_P] * 9,
@@ -115,6 +134,11 @@ def call(name, *args):
 def dptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(_P)
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_FP)
 
 
 def iptr(a: np.ndarray):

@@ -16,13 +16,13 @@
 #include <algorithm>
 #include <vector>
 #include "fipc_ctx.h"
+#include "fipc_rows.h"
 
 using namespace fipc;
 
 #define TPB 256
 static inline int nblk(long n, int t = TPB) { return (int)((n + t - 1) / t); }
 
-__device__ __forceinline__ v3 ldv(const double* a, int i) { return v3{a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
 
 struct CellRange { int lo[3], hi[3]; };
 __device__ __forceinline__ CellRange cells_of(v3 lo, v3 hi, double inv)

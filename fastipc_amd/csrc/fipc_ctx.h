@@ -75,8 +75,8 @@ struct Grid {
 
 // HIP-event timing of single kernels: pairs are recorded on the context
 // stream without synchronising and folded into totals when read
-#define FIPC_NTIMERS 9
-enum { TM_SPMV = 0, TM_ELASTIC, TM_CCD, TM_FIND, TM_ENERGY, TM_CONTACT, TM_PCGVEC, TM_GRID, TM_GATHER };
+#define FIPC_NTIMERS 10
+enum { TM_SPMV = 0, TM_ELASTIC, TM_CCD, TM_FIND, TM_ENERGY, TM_CONTACT, TM_PCGVEC, TM_GRID, TM_GATHER, TM_SPMV_RES };
 struct KernelTimer {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     double total_ms = 0.0;
@@ -127,6 +127,15 @@ struct fipc_ctx {
     DVec<int2> ccd_pairs[2];
     // broad phase grids
     Grid grid;
+    // learned warm start
+    DVec<double> dprev;    // previous step's correction x_{n} - xTilde_{n-1} (0 on Dirichlet vertices)
+    DVec<float> wsX;       // features, nv x 16
+    DVec<unsigned long long> ckey;  // nearest contact row per vertex
+    DVec<int> free_ids;    // vertices that are not Dirichlet (built on first capture)
+    int nfree = -1;
+    DVec<int> cap_ids;     // vertices captured at the start of the step
+    DVec<float> cap_X, cap_T;
+    int cap_n = 0;
     // statistics
     double last_pcg_relres = 0.0;
     int last_pcg_iters = 0;

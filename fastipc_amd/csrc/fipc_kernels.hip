@@ -15,6 +15,7 @@
 #include <cstring>
 #include <vector>
 #include "fipc_ctx.h"
+#include "fipc_rows.h"
 
 using namespace fipc;
 
@@ -77,7 +78,6 @@ __device__ __forceinline__ double reduce_partials(const double* part, int n)
     __syncthreads();
     return r;
 }
-__device__ __forceinline__ v3 ldv(const double* a, int i) { return v3{a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
 
 // Cross-block reductions never synchronise blocks inside a kernel.  On gfx950
 // the L2 is private to each XCD, so an agent-scope release / acquire pair
@@ -112,10 +112,14 @@ __global__ void k_begin_step(int nv, const double* __restrict__ x, const double*
     }
 }
 // compute_v (NEWTON_IPC.py:799-802)
-__global__ void k_end_step(int n3, const double* x, const double* xn, double* v, double inv_dt)
+// (also keeps x - xTilde of free vertices, a feature of the learned warm start)
+__global__ void k_end_step(int n3, const double* x, const double* xn, const double* xt, const unsigned char* dmask,
+                           double* v, double* dprev, double inv_dt)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n3) v[i] = (x[i] - xn[i]) * inv_dt;
+    if (i >= n3) return;
+    v[i] = (x[i] - xn[i]) * inv_dt;
+    dprev[i] = dmask[i / 3] ? 0.0 : x[i] - xt[i];
 }
 // apply_sol (NEWTON_IPC.py:792-796)
 __global__ void k_apply(int n3, const double* xp, const double* p, double* x, double a)
@@ -255,27 +259,6 @@ __global__ void k_gather_rhs(int nv, const int* __restrict__ vcptr, const int* _
 // Every lane evaluates the constraint and runs the Jacobi sweeps on its own
 // copy of the packed Hessian; lane j owns eigenvector rows 3j..3j+2 and
 // writes rows 3j..3j+2 of the projected Hessian (makePD semantics, UTILS.h:7-25).
-template <int KIND>
-struct KindInfo {
-    static constexpr int K = kind_points(KIND);
-    static constexpr int N = 3 * K;
-    static constexpr int G = K == 2 ? 2 : 4;
-};
-
-template <int KIND>
-__device__ __forceinline__ void load_row(const int* rows, int r, const double* x, const double* x0, int (&id)[4], v3 (&px)[4], v3 (&p0)[4])
-{
-    constexpr int K = KindInfo<KIND>::K;
-    constexpr int W = KIND == 0 ? 2 : (KIND == 1 ? 3 : 4);
-#pragma unroll
-    for (int a = 0; a < 4; a++) id[a] = a < K ? rows[W * r + a] : 0;
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-        px[a] = a < K ? ldv(x, id[a]) : mk(0, 0, 0);
-        p0[a] = (KIND >= 4 && a < K) ? ldv(x0, id[a]) : mk(0, 0, 0);
-    }
-}
-
 template <int KIND>
 __global__ void __launch_bounds__(TPB) k_contact(int n, const int* __restrict__ rows, const double* __restrict__ x,
                                                  const double* __restrict__ x0, double dHat2, double kappa,
@@ -823,6 +806,8 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->grid.cnt); release(c->grid.ekey); release(c->grid.eitem); release(c->grid.bstart);
     release(c->grid.bfill); release(c->grid.items); release(c->grid.ecell); release(c->grid.cells);
     release(c->grid.qoff); release(c->ccd_pairs[0]); release(c->ccd_pairs[1]);
+    release(c->dprev); release(c->wsX); release(c->ckey); release(c->free_ids); release(c->cap_ids);
+    release(c->cap_X); release(c->cap_T);
     (void)fipc_tm_fold(c);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->s);
@@ -931,6 +916,9 @@ extern "C" int fipc_upload_mesh(fipc_ctx* c, int nv, const double* x0, const dou
     HIPCHK(hipMemcpyAsync(c->x.p, x0, n3 * 8, hipMemcpyHostToDevice, c->s));
     HIPCHK(hipMemcpyAsync(c->xt.p, x0, n3 * 8, hipMemcpyHostToDevice, c->s));
     HIPCHK(hipMemsetAsync(c->v.p, 0, n3 * 8, c->s));
+    STCHK(ensure(c->dprev, n3));
+    HIPCHK(hipMemsetAsync(c->dprev.p, 0, n3 * 8, c->s));
+    c->nfree = -1;
     HIPCHK(hipMemsetAsync(c->sol.p, 0, n3 * 8, c->s));
     HIPCHK(hipMemsetAsync(c->rhs.p, 0, n3 * 8, c->s));
     int nmax = std::max(std::max(nv, nt), 4096);
@@ -968,6 +956,8 @@ extern "C" int fipc_set_state(fipc_ctx* c, const double* x, const double* v)
     size_t n = (size_t)c->nv * 3 * 8;
     if (x) HIPCHK(hipMemcpyAsync(c->x.p, x, n, hipMemcpyHostToDevice, c->s));
     if (v) HIPCHK(hipMemcpyAsync(c->v.p, v, n, hipMemcpyHostToDevice, c->s));
+    // a new (x, v) state has no previous step
+    if (v) HIPCHK(hipMemsetAsync(c->dprev.p, 0, n, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     return FIPC_OK;
 }
@@ -1023,7 +1013,8 @@ extern "C" int fipc_begin_step(fipc_ctx* c, const double* target)
 extern "C" int fipc_end_step(fipc_ctx* c)
 {
     if (!c || !c->nv) return FIPC_ERR_STATE;
-    k_end_step<<<nblk(c->nv * 3), TPB, 0, c->s>>>(c->nv * 3, c->x.p, c->xn.p, c->v.p, 1.0 / c->dt);
+    k_end_step<<<nblk(c->nv * 3), TPB, 0, c->s>>>(c->nv * 3, c->x.p, c->xn.p, c->xt.p, c->dmask.p, c->v.p, c->dprev.p,
+                                                   1.0 / c->dt);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->s));
     return FIPC_OK;
@@ -1130,7 +1121,10 @@ static int spmv(fipc_ctx* c, int prec, const double* p, double* q, double* qc, d
     ContactSet cs;
     int nbc = spmv_contacts(c, cs);
     int nbb = spmv_grid(c->nv);
-    STCHK(fipc_tm_begin(c, TM_SPMV));
+    // products inside the PCG iterations and the residual products (initial
+    // guess, FP64 refinement) are timed as separate classes
+    const int timer = part ? TM_SPMV : TM_SPMV_RES;
+    STCHK(fipc_tm_begin(c, timer));
     if (prec == 1)
         k_spmv<float><<<nbc + nbb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->valf.p, p, q, qc, c->dmask.p,
                                                    cs, nbc, nbb, part, state, it);
@@ -1138,7 +1132,7 @@ static int spmv(fipc_ctx* c, int prec, const double* p, double* q, double* qc, d
         k_spmv<double><<<nbc + nbb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->val.p, p, q, qc, c->dmask.p,
                                                     cs, nbc, nbb, part, state, it);
     HIPCHK(hipGetLastError());
-    STCHK(fipc_tm_end(c, TM_SPMV));
+    STCHK(fipc_tm_end(c, timer));
     return FIPC_OK;
 }
 // partials of one product (contact blocks, padded to a multiple of 8, then BSR blocks)

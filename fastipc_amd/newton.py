@@ -43,6 +43,7 @@ class StepStats:
     counts: list = field(default_factory=list)
     energy: float = float("nan")
     converged: bool = True
+    warm_accepted: bool = False
 
 
 class NewtonIPC:
@@ -185,20 +186,26 @@ class NewtonIPC:
         _lib.call("fipc_end_step", self.ctx)
 
     # --------------------------------------------------------------- frame loop
-    def step(self, warm_start=None) -> StepStats:
+    def step(self, warm_start=None, capture=None) -> StepStats:
         """One time step (NEWTON_IPC.py:956-999).  ``warm_start`` optionally
-        returns a displacement increment to try before the first Newton
-        iteration; it is kept only if it is intersection-free and lowers the
-        incremental potential, otherwise x = x_n is used as in the reference."""
+        proposes an increment before the first Newton iteration (a callable
+        that returns a host array, or True after writing the device direction
+        vector); it is kept only if it is intersection-free and lowers the
+        incremental potential, otherwise x = x_n is used as in the reference.
+        ``capture`` (fastipc_amd.warmstart.Capture) records training rows of
+        the step."""
         st = StepStats()
         T = self.timers
         f = self.frame
         dt = self.setup.dt
         with T("step"):
             self.compute_xn_and_xTilde(f * dt)
+            if capture is not None:
+                self.find_constraints()
+                capture.begin(self)
             if warm_start is not None:
                 with T("warm_start"):
-                    self._try_warm_start(warm_start)
+                    st.warm_accepted = self._try_warm_start(warm_start)
             while True:
                 st.newton_iters += 1
                 with T("build"):
@@ -219,6 +226,8 @@ class NewtonIPC:
                 with T("line_search"):
                     alpha = self._line_search()
                     st.alphas.append(alpha)
+            if capture is not None:
+                capture.end(self)
             self.compute_v()
         st.energy = float("nan")
         self.frame += 1
@@ -260,20 +269,34 @@ class NewtonIPC:
                 n += 1
         return alpha
 
-    def _try_warm_start(self, fn):
-        dx = fn(self)
-        if dx is None:
-            return
-        x_n = self.x
+    def _try_warm_start(self, fn, max_halvings: int = 4) -> bool:
+        """Line search on the proposed increment (the steps of :980-995):
+        CCD / inversion bound, halve on collision, then halve while the
+        incremental potential is not below its value at x_n; if no halving
+        succeeds, x = x_n."""
         self.find_constraints()
         E0 = self.compute_energy()
         self.save_xPrev()
-        self.set_vector(1, dx)
+        dx = fn(self)
+        if dx is None or dx is False:
+            return False
+        if dx is not True:
+            self.set_vector(1, np.ascontiguousarray(dx, np.float64).reshape(-1, 3))
         alpha = min(1.0, self.compute_intersection_free_step_size())
         self.apply_sol(alpha)
-        if self.check_collision():
-            self.set_state(x=x_n)
-            return
-        self.find_constraints()
-        if self.compute_energy() > E0:
-            self.set_state(x=x_n)
+        n = 0
+        while self.check_collision():
+            n += 1
+            if n > self.max_halvings:
+                self.apply_sol(0.0)
+                return False
+            alpha /= 2.0
+            self.apply_sol(alpha)
+        for _ in range(max_halvings + 1):
+            self.find_constraints()
+            if self.compute_energy() < E0:
+                return True
+            alpha *= 0.5
+            self.apply_sol(alpha)
+        self.apply_sol(0.0)
+        return False

@@ -165,10 +165,11 @@ int fipc_get_bsr(fipc_ctx* c, int* nnzb, int* rowptr, int* col, double* val);
 int fipc_spmv(fipc_ctx* c, const double* x, double* y);
 /* timing of the hot kernels with HIP events on the context stream, recorded
  * without synchronisation while enabled and folded when read.  Classes:
- * 0 BSR SpMV, 1 per-tet elastic pass, 2 CCD query, 3 constraint / collision
- * query, 4 energy, 5 contact assembly, 6 PCG vector updates, 7 grid build
- * (phase time on the stream), 8 assembly gathers.  Writes the first n (<= 9)
- * totals (ms) and launch counts. */
+ * 0 SpMV inside the PCG iterations (contact + BSR rows), 1 per-tet elastic
+ * pass, 2 CCD query, 3 constraint / collision query, 4 energy, 5 contact
+ * assembly, 6 PCG vector updates, 7 grid build (phase time on the stream),
+ * 8 assembly gathers, 9 residual SpMVs (initial guess, FP64 refinement).
+ * Writes the first n (<= 10) totals (ms) and launch counts. */
 int fipc_enable_timing(fipc_ctx* c, int on);
 int fipc_kernel_times(fipc_ctx* c, int n, double* ms, long* launches);
 int fipc_sync(fipc_ctx* c);
@@ -177,6 +178,61 @@ int fipc_sync(fipc_ctx* c);
  * bandwidth reference and the calibration kernel of hardware byte counters
  * (no counterpart in the reference). */
 int fipc_stream_copy(fipc_ctx* c, long nbytes, double* ms);
+
+/* ---------------------------------------------------------------------- */
+/* 3. learned Newton warm start.  No counterpart in the reference: its only */
+/*    contact point is the initial guess x = x_n of NEWTON_IPC.py:958-960,  */
+/*    which stays the fallback.                                             */
+/* ---------------------------------------------------------------------- */
+typedef struct fipc_mlp fipc_mlp;
+/* per-vertex network n_in -> depth x width (SiLU) -> n_out (linear); products
+ * on the BF16 matrix cores with FP32 accumulation, FP32 master parameters
+ * packed per layer as W [fan_in][fan_out] then b [fan_out]; initialised from
+ * seed (variance 1/fan_in, output layer 10x smaller) */
+int fipc_mlp_create(fipc_mlp** out, int n_in, int width, int depth, int n_out, unsigned long long seed);
+int fipc_mlp_destroy(fipc_mlp* m);
+int fipc_mlp_nparams(fipc_mlp* m, long* n);
+int fipc_mlp_get_params(fipc_mlp* m, float* w);
+/* sets the parameters and resets the Adam state */
+int fipc_mlp_set_params(fipc_mlp* m, const float* w);
+/* Y[n][n_out] = network(X[n][n_in]) (host arrays) */
+int fipc_mlp_forward(fipc_mlp* m, int n, const float* X, float* Y);
+/* mean squared error over n rows and its gradient (nparams, packed like the
+ * parameters; grad may be NULL) */
+int fipc_mlp_grad(fipc_mlp* m, int n, const float* X, const float* T, float* loss, float* grad);
+/* device-resident training set of (feature, target) rows */
+int fipc_mlp_dataset_clear(fipc_mlp* m);
+int fipc_mlp_dataset_size(fipc_mlp* m, long* n);
+int fipc_mlp_dataset_add(fipc_mlp* m, int n, const float* X, const float* T);
+int fipc_mlp_dataset_get(fipc_mlp* m, long first, long n, float* X, float* T);
+/* iters Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on batches of `batch`
+ * rows drawn from the training set; losses[iters] (may be NULL) = mean
+ * squared error of each batch before its update */
+int fipc_mlp_train(fipc_mlp* m, int iters, int batch, float lr, unsigned long long seed, float* losses);
+
+/* Warm start on a context, after fipc_begin_step and fipc_find_constraints
+ * at x_n.  Features per vertex (16, scaled by `scale`, a length):
+ *   0-2 v dt, 3-5 the previous step's correction x_n - xTilde_{n-1},
+ *   6-8 the gravity increment dt^2 g, 9 nearest contact distance / dHat
+ *   (1 without contact), 10-12 unit normal away from that contact (0 without),
+ *   13 Dirichlet flag, 14-15 the contact partner's inertial (or prescribed)
+ *   increment relative to the vertex's own: approach along the normal and
+ *   tangential size (0 without contact).
+ * fipc_ws_features copies them to feat[nv][16] (may be NULL).
+ * fipc_ws_predict writes the increment
+ *   p = xTilde - x_n + c_prev + scale * network(f),
+ * c_prev = the previous step's correction (features 3-5 times scale), i.e.
+ * the network predicts the change of the correction; Dirichlet vertices get
+ * their prescribed increment.  p is the solution vector, to be filtered by
+ * fipc_ccd_alpha and applied with fipc_apply_sol.
+ * fipc_ws_capture_begin keeps the features of nsample free vertices drawn
+ * from seed; fipc_ws_capture_end (after the step, before fipc_end_step)
+ * appends them to the training set with targets
+ * ((x_{n+1} - xTilde) - c_prev) / scale. */
+int fipc_ws_features(fipc_ctx* c, double scale, float* feat);
+int fipc_ws_predict(fipc_ctx* c, fipc_mlp* m, double scale);
+int fipc_ws_capture_begin(fipc_ctx* c, fipc_mlp* m, double scale, int nsample, unsigned long long seed);
+int fipc_ws_capture_end(fipc_ctx* c, fipc_mlp* m, double scale);
 
 #ifdef __cplusplus
 }
