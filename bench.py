@@ -11,8 +11,8 @@ each ending with a device synchronisation.  The position state is resident in
 HBM for the whole run.
 
 Printed: one JSON line with steps/s, the roofline figures of the dominant
-kernel (HIP-event durations of its launches over the timed steps, against its
-algorithmic bytes; the HBM traffic per launch measured with rocprofv3
+kernel (HIP-event durations of its launches over --roofline-steps further
+steps of the same run, against its algorithmic bytes; the HBM traffic per launch measured with rocprofv3
 FETCH_SIZE / WRITE_SIZE, read from profiles/k_spmv_traffic_compressed_slab.md)
 and a CPU baseline (the C oracle, OpenMP, with a direct sparse solve) on a
 bounded sample of the same scene type.
@@ -81,7 +81,7 @@ def run_gpu(args):
     from fastipc_amd import scene as S
     from fastipc_amd.newton import NewtonIPC
 
-    sc = S.compressed_slab(50, 80, 50, steps=args.warmup + args.steps + 1)
+    sc = S.compressed_slab(50, 80, 50, steps=args.warmup + args.steps + args.roofline_steps)
     precision = 1 if args.mode == "fp32" else 0
     t0 = time.perf_counter()
     sim = NewtonIPC(sc, precision=precision, pcg_tol=args.pcg_tol)
@@ -93,10 +93,6 @@ def run_gpu(args):
         log(f"warmup step {i}: {time.perf_counter() - t1:.3f}s newton {st.newton_iters} pcg {st.pcg_iters} "
             f"contacts {st.counts[-1] if st.counts else None}")
     _lib.call("fipc_sync", sim.ctx)
-    # HIP-event timing of the kernel classes over the timed region (events are
-    # recorded on the context stream without synchronising)
-    if not args.traffic_probe:
-        _lib.call("fipc_enable_timing", sim.ctx, 1)
     t0 = time.perf_counter()
     pcg = []
     for i in range(args.steps):
@@ -117,6 +113,16 @@ def run_gpu(args):
             f"({2 * (256 << 20) / cms.value / 1e6:.0f} GB/s)")
         sim.close()
         return None
+    # kernel durations: HIP events recorded on the context stream around every
+    # launch of the timed classes, over further steps of the same run (so that
+    # the event records do not perturb the steps/s above)
+    _lib.call("fipc_enable_timing", sim.ctx, 1)
+    for i in range(args.roofline_steps):
+        st = sim.step()
+        log(f"event-timed step {i}: newton {st.newton_iters} pcg {st.pcg_iters}")
+        if sim.check_collision():
+            raise RuntimeError("intersection after a step")
+    _lib.call("fipc_sync", sim.ctx)
     nt_ = len(TIMER_NAMES)
     ms = np.zeros(nt_)
     la = (C.c_long * nt_)()
@@ -148,7 +154,7 @@ def run_gpu(args):
         "contacts_last_step": {n: int(c) for n, c in zip(("PP", "PE", "PT", "EE", "EEM", "PPM", "PEM"), counts)},
         "roofline": roof,
         "kernel_classes": others,
-        "kernel_times_timed_region": kt,
+        "kernel_times_event_steps": kt,
         "setup_s": setup_s,
         "nv": sim.nv,
         "nt": len(sc.tets),
@@ -200,6 +206,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-warmup", type=int, default=6)
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
+    ap.add_argument("--roofline-steps", type=int, default=5,
+                    help="steps after the timed ones whose kernels are timed with HIP events")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="counter-calibration run for rocprofv3 --pmc: the timed steps, then a 256 MiB "
                          "device copy instead of the timing step; prints no JSON line")
@@ -233,6 +241,8 @@ def main():
         "roofline": g["roofline"],
         "cpu_baseline": cpu,
         "kernel_classes": g["kernel_classes"],
+        "kernel_times": {k: {"launches": v["launches"], "us_avg": round(v["us_avg"], 2)}
+                         for k, v in g["kernel_times_event_steps"].items() if v["launches"]},
     }
     print(json.dumps(line))
 

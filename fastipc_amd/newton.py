@@ -98,7 +98,28 @@ class NewtonIPC:
         _lib.call("fipc_get_state", self.ctx, None, _lib.dptr(out))
         return out
 
+    # The constraint set and the energy of the current x are kept until x (or
+    # xTilde, for the energy) changes, so the frame loop does not recompute
+    # what the line search has just computed at the same x: the Newton
+    # iteration reuses the constraints found after the last step of the line
+    # search, and the next line search starts from the energy evaluated there
+    # (same x, same constraints, deterministic reduction: the same bits).
+    def _invalidate(self, x_changed: bool = True):
+        if x_changed:
+            self._cons_ok = False
+        self._E = None
+
+    def _ensure_constraints(self):
+        if not getattr(self, "_cons_ok", False):
+            self.find_constraints()
+
+    def _energy(self) -> float:
+        if getattr(self, "_E", None) is None:
+            self.compute_energy()
+        return self._E
+
     def set_state(self, x=None, v=None):
+        self._invalidate()
         xa = _lib.f64(x) if x is not None else None
         va = _lib.f64(v) if v is not None else None
         _lib.call("fipc_set_state", self.ctx, _lib.dptr(xa) if xa is not None else None,
@@ -110,6 +131,8 @@ class NewtonIPC:
         return out
 
     def set_vector(self, which: int, a):
+        if which in (0, 3):
+            self._invalidate(x_changed=which == 0)
         a = _lib.f64(a).reshape(self.nv, 3)
         _lib.call("fipc_set_vector", self.ctx, which, _lib.dptr(a))
 
@@ -118,11 +141,14 @@ class NewtonIPC:
         """compute_xn_and_xTilde + move_nodes (NEWTON_IPC.py:497-511)."""
         target = _lib.f64(self.scene.dirichlet(t))
         _lib.call("fipc_begin_step", self.ctx, _lib.dptr(target))
+        self._invalidate(x_changed=False)  # xTilde changed: the energy, not the constraints
 
     def find_constraints(self):
         counts = np.zeros(7, np.int32)
         _lib.call("fipc_find_constraints", self.ctx, _lib.iptr(counts))
         self.counts = [int(c) for c in counts]
+        self._cons_ok = True
+        self._E = None
         return self.counts
 
     def constraints(self, sort: bool = True):
@@ -163,6 +189,7 @@ class NewtonIPC:
         e = C.c_double()
         p = np.zeros(3)
         _lib.call("fipc_energy", self.ctx, C.byref(e), _lib.dptr(p))
+        self._E = e.value
         return (e.value, p) if parts else e.value
 
     def compute_intersection_free_step_size(self, parts: bool = False):
@@ -181,6 +208,7 @@ class NewtonIPC:
 
     def apply_sol(self, alpha: float):
         _lib.call("fipc_apply_sol", self.ctx, float(alpha))
+        self._invalidate()
 
     def compute_v(self):
         _lib.call("fipc_end_step", self.ctx)
@@ -201,7 +229,7 @@ class NewtonIPC:
         with T("step"):
             self.compute_xn_and_xTilde(f * dt)
             if capture is not None:
-                self.find_constraints()
+                self._ensure_constraints()
                 capture.begin(self)
             if warm_start is not None:
                 with T("warm_start"):
@@ -210,7 +238,8 @@ class NewtonIPC:
                 st.newton_iters += 1
                 with T("build"):
                     with T("find"):
-                        st.counts.append(self.find_constraints())
+                        self._ensure_constraints()
+                        st.counts.append(list(self.counts))
                     with T("assemble"):
                         self.compute_hessian_and_gradient()
                 with T("solve"):
@@ -237,7 +266,7 @@ class NewtonIPC:
     def _line_search(self) -> float:
         T = self.timers
         with T("energy"):
-            E0 = self.compute_energy()
+            E0 = self._energy()
         self.save_xPrev()
         with T("ccd"):
             alpha = self.compute_intersection_free_step_size()
@@ -274,8 +303,8 @@ class NewtonIPC:
         CCD / inversion bound, halve on collision, then halve while the
         incremental potential is not below its value at x_n; if no halving
         succeeds, x = x_n."""
-        self.find_constraints()
-        E0 = self.compute_energy()
+        self._ensure_constraints()
+        E0 = self._energy()
         self.save_xPrev()
         dx = fn(self)
         if dx is None or dx is False:

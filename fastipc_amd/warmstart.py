@@ -179,26 +179,30 @@ class Capture:
 # ----------------------------------------------------------------------------
 # scene families: the training variations and the held-out evaluation scenes
 # ----------------------------------------------------------------------------
-def family_scene(family: str, seed: int, size: str = "train"):
+def family_scene(family: str, seed: int, size: str = "full"):
+    """Training and evaluation scenes.  slab: the compressed slab with seeded
+    closing rate (0.1-0.3 heights/s), twist amplitude (10-45 deg) and frequency
+    (0.5-1.5 Hz) at 20x32x20 ("small"), 30x48x30 ("eval") or 50x80x50
+    ("full") vertices; "bench" is the 50x80x50 benchmark scene itself (0.2,
+    30 deg, 1 Hz), held out of training.  cubes: the 3x3x3 grid of cubes with
+    seeded rotations and jitter, 8^3-vertex cubes or 16^3 ("bench")."""
     from . import scene as S
     rng = np.random.default_rng(seed)
     if family == "slab":
-        # compressed slab with seeded closing rate, twist amplitude and
-        # frequency; held-out evaluation uses the benchmark's own settings
         if size == "bench":
             return S.compressed_slab(50, 80, 50)
         close = rng.uniform(0.1, 0.3)
         twist = rng.uniform(10.0, 45.0)
         omega = 2 * math.pi * rng.uniform(0.5, 1.5)
-        dims = (20, 32, 20) if size == "train" else (30, 48, 30)
+        dims = {"small": (20, 32, 20), "eval": (30, 48, 30), "full": (50, 80, 50)}[size]
         return S.compressed_slab(*dims, close_rate=close, twist_deg=twist, omega=omega)
     if family == "cubes":
-        n = 16 if size == "bench" else 8
+        n = 16 if size in ("bench", "full") else 8
         return S.cube_grid(n, 3, seed=seed)
     raise ValueError(family)
 
 
-def capture_trajectories(mlp, family, seeds, steps, nsample=4096, size="train", log=print):
+def capture_trajectories(mlp, family, seeds, steps, nsample=4096, size="full", log=print):
     from .newton import NewtonIPC
     stats = []
     for sd in seeds:
@@ -226,7 +230,7 @@ def error_stats(P, T, tol=0.1):
             "p999": float(np.quantile(e, 0.999)), "max": float(e.max()), "frac_above_tol": float(np.mean(e > tol))}
 
 
-def validation(mlp, family, seed, steps, size="eval", nsample=16384, skip=10, log=print):
+def validation(mlp, family, seed, steps, size="bench", nsample=16384, skip=10, log=print):
     """Held-out trajectory of the plain solver: errors of the zero network
     (second-order extrapolation) and of the trained network."""
     from .newton import NewtonIPC
@@ -296,13 +300,15 @@ def main(argv=None):
     tr.add_argument("--iters", type=int, default=3000)
     tr.add_argument("--batch", type=int, default=8192)
     tr.add_argument("--lr", type=float, default=1e-3)
+    tr.add_argument("--size", default="full")
+    tr.add_argument("--val-size", default="bench")
     tr.add_argument("--val-seed", type=int, default=1000)
     tr.add_argument("--out", required=True)
     ev = sub.add_parser("eval")
     ev.add_argument("--family", choices=("slab", "cubes"), default="slab")
     ev.add_argument("--weights", required=True)
     ev.add_argument("--seeds", default="1000-1001")
-    ev.add_argument("--size", default="eval")
+    ev.add_argument("--size", default="bench")
     ev.add_argument("--steps", type=int, default=20)
     ev.add_argument("--warmup", type=int, default=10)
     ev.add_argument("--precision", type=int, default=0)
@@ -318,7 +324,7 @@ def main(argv=None):
     if a.cmd == "train":
         mlp = WarmStartMLP(seed=12345)
         t0 = time.perf_counter()
-        capture_trajectories(mlp, a.family, seeds(a.seeds), a.steps, a.nsample, log=log)
+        capture_trajectories(mlp, a.family, seeds(a.seeds), a.steps, a.nsample, size=a.size, log=log)
         t1 = time.perf_counter()
         losses = []
         chunk = 250
@@ -329,11 +335,11 @@ def main(argv=None):
         t2 = time.perf_counter()
         X, T = mlp.dataset()
         base = float(np.mean(T ** 2))
-        val = validation(mlp, a.family, a.val_seed, a.steps, log=log)
+        val = validation(mlp, a.family, a.val_seed, a.steps, size=a.val_size, log=log)
         meta = {"family": a.family, "seeds": a.seeds, "steps": a.steps, "nsample": a.nsample, "iters": a.iters,
                 "batch": a.batch, "lr": a.lr, "rows": int(len(X)), "final_loss": float(np.mean(losses[-100:])),
                 "zero_prediction_loss": base, "capture_s": t1 - t0, "train_s": t2 - t1,
-                "validation_seed": a.val_seed, "validation": val}
+                "size": a.size, "validation_seed": a.val_seed, "validation_size": a.val_size, "validation": val}
         mlp.save(a.out, meta)
         log(json.dumps(meta))
     else:
