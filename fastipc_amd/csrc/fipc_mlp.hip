@@ -217,6 +217,99 @@ __global__ void k_gather_batch(int B, long nds, unsigned long long seed, int nin
 }
 
 // ---------------------------------------------------------------------------
+// Fused inference for the warm-start shape (16 -> 4 x 256 SiLU -> 3): a block
+// of 4 waves takes 128 rows through all five layers; the activations stay in
+// LDS as BF16 (the products' input precision) and never touch HBM.  Weights
+// are read from L2 as BF16 transposed per layer, Wt[n][k] (the last layer
+// padded to 32 rows of zeros), so a lane's B fragment (8 consecutive k of one
+// output column) is one 16-byte load.  A block has 8 waves; wave w owns output
+// columns 64 (w % 4) .. +64 (two 32-column MFMA tiles) for rows 64 (w / 4) ..
+// +64 (two 32-row tiles): each weight fragment feeds two MFMAs, each
+// activation fragment (from LDS) two, and 64 accumulators per lane leave
+// room for two waves per SIMD.  The accumulation order over k is the generic
+// GEMM's.
+// ---------------------------------------------------------------------------
+#define FI_ROWS 128
+#define FI_W 256
+#define FI_LDS (FI_W + 8)
+__global__ void __launch_bounds__(512) k_mlp_infer(int n, const float* __restrict__ X, const __bf16* __restrict__ Wt,
+                                                   const float* __restrict__ params, const long* __restrict__ boff,
+                                                   float* __restrict__ Y)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 H[FI_ROWS][FI_LDS];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int cq = w & 3, rh = w >> 2;
+    const long r0 = (long)blockIdx.x * FI_ROWS;
+    // input tile: 128 rows x 16 features
+    for (int e = tid; e < FI_ROWS * 16; e += blockDim.x) {
+        int r = e >> 4, k = e & 15;
+        long gr = r0 + r;
+        H[r][k] = (__bf16)(gr < n ? X[gr * 16 + k] : 0.f);
+    }
+    __syncthreads();
+    const __bf16* W = Wt;
+    for (int l = 0; l < 5; l++) {
+        const int fi = l == 0 ? 16 : FI_W;
+        const int ntile = l == 4 ? (cq == 0 ? 1 : 0) : 2;
+        floatx16 acc[2][2];
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+#pragma unroll
+                for (int j = 0; j < 16; j++) acc[rt][t][j] = 0.f;
+        if (ntile > 0) {
+            for (int k0 = 0; k0 < fi; k0 += 16) {
+                bf16x8 b[2];
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    int col = cq * 64 + t * 32 + (lane & 31);
+                    if (t < ntile) b[t] = *(const bf16x8*)&W[(long)col * fi + k0 + (lane >> 5) * 8];
+                }
+#pragma unroll
+                for (int rt = 0; rt < 2; rt++) {
+                    bf16x8 a = *(const bf16x8*)&H[rh * 64 + rt * 32 + (lane & 31)][k0 + (lane >> 5) * 8];
+#pragma unroll
+                    for (int t = 0; t < 2; t++)
+                        if (t < ntile) acc[rt][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[t], acc[rt][t], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();  // every wave has read this layer's input
+        const float* bias = params + boff[l];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            if (t >= ntile) continue;
+            int col = cq * 64 + t * 32 + (lane & 31);
+            float bc = (l < 4 || col < 3) ? bias[col] : 0.f;
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    int row = rh * 64 + rt * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+                    float z = acc[rt][t][j] + bc;
+                    if (l < 4) {
+                        H[row][col] = (__bf16)(z * sigm(z));
+                    } else if (col < 3) {
+                        long gr = r0 + row;
+                        if (gr < n) Y[gr * 3 + col] = z;
+                    }
+                }
+        }
+        __syncthreads();
+        W += (long)(l == 4 ? 32 : FI_W) * fi;
+    }
+}
+// Wt (BF16, transposed, last layer padded to 32 rows) from the FP32 parameters
+__global__ void k_pack_wt(int fi, int fo, int rows, const float* __restrict__ W, __bf16* __restrict__ Wt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * fi) return;
+    int nrow = i / fi, k = i % fi;
+    Wt[i] = (__bf16)(nrow < fo ? W[(long)k * fo + nrow] : 0.f);
+}
+
+// ---------------------------------------------------------------------------
 // the network object
 // ---------------------------------------------------------------------------
 struct fipc_mlp {
@@ -230,6 +323,12 @@ struct fipc_mlp {
     int tcap = 0;                  // rows the training buffers (Z, gradients, X, T) hold
     std::vector<DVec<float>> H, Z; // H[l] (l >= 1) layer outputs, Z[l] hidden pre-activations
     DVec<float> X, T, dA, dB, part, loss;
+    // fused inference (warm-start shape only): packed BF16 weights, stale
+    // after any parameter change
+    bool fused = false;
+    bool wt_ok = false;
+    DVec<__bf16> wt;
+    DVec<long> dboff;
     // dataset
     DVec<float> dsX, dsT;
     long ds_n = 0;
@@ -311,11 +410,41 @@ int mlp_dims(const fipc_mlp* m, int* n_in, int* n_out)
     *n_out = m->n_out;
     return FIPC_OK;
 }
-int mlp_forward_dev(fipc_mlp* m, hipStream_t s, int n, const float* X, const float** Y)
+static int mlp_pack(fipc_mlp* m, hipStream_t s)
 {
-    STCHK(mlp_forward(m, s, n, X, false));
+    if (m->wt_ok) return FIPC_OK;
+    long total = 0;
+    for (int l = 0; l < m->L; l++) total += (long)(l == m->L - 1 ? 32 : m->dim[l + 1]) * m->dim[l];
+    STCHK(ensure(m->wt, total));
+    long o = 0;
+    for (int l = 0; l < m->L; l++) {
+        int fi = m->dim[l], fo = m->dim[l + 1], rows = l == m->L - 1 ? 32 : fo;
+        k_pack_wt<<<nblk((long)rows * fi), TPB, 0, s>>>(fi, fo, rows, m->params.p + m->woff[l], m->wt.p + o);
+        o += (long)rows * fi;
+    }
+    HIPCHK(hipGetLastError());
+    m->wt_ok = true;
+    return FIPC_OK;
+}
+// forward of n rows of device X on stream s: the fused kernel for the
+// warm-start shape, the layer-by-layer GEMMs otherwise
+static int mlp_infer(fipc_mlp* m, hipStream_t s, int n, const float* X, const float** Y)
+{
+    if (!m->fused) {
+        STCHK(mlp_forward(m, s, n, X, false));
+        *Y = m->H[m->L].p;
+        return FIPC_OK;
+    }
+    STCHK(mlp_pack(m, s));
+    STCHK(ensure(m->H[m->L], (size_t)n * m->n_out));
+    k_mlp_infer<<<(n + FI_ROWS - 1) / FI_ROWS, 512, 0, s>>>(n, X, m->wt.p, m->params.p, m->dboff.p, m->H[m->L].p);
+    HIPCHK(hipGetLastError());
     *Y = m->H[m->L].p;
     return FIPC_OK;
+}
+int mlp_forward_dev(fipc_mlp* m, hipStream_t s, int n, const float* X, const float** Y)
+{
+    return mlp_infer(m, s, n, X, Y);
 }
 int mlp_dataset_append_dev(fipc_mlp* m, hipStream_t s, int n, const float* X, const float* T)
 {
@@ -383,6 +512,11 @@ extern "C" int fipc_mlp_create(fipc_mlp** out, int n_in, int width, int depth, i
             nw, m->dim[l + 1], m->params.p + m->woff[l], m->params.p + m->boff[l], a, seed + 7919ull * l);
     }
     if (hipGetLastError() != hipSuccess) { fipc_mlp_destroy(m); return FIPC_ERR_HIP; }
+    m->fused = n_in == 16 && width == FI_W && depth == 4 && n_out == 3;
+    if (m->fused) {
+        if (ensure(m->dboff, m->L) != FIPC_OK) { fipc_mlp_destroy(m); return FIPC_ERR_HIP; }
+        (void)hipMemcpyAsync(m->dboff.p, m->boff.data(), m->L * sizeof(long), hipMemcpyHostToDevice, m->s);
+    }
     (void)hipMemsetAsync(m->am.p, 0, off * 4, m->s);
     (void)hipMemsetAsync(m->av.p, 0, off * 4, m->s);
     if (hipStreamSynchronize(m->s) != hipSuccess) { fipc_mlp_destroy(m); return FIPC_ERR_HIP; }
@@ -398,7 +532,7 @@ extern "C" int fipc_mlp_destroy(fipc_mlp* m)
     for (auto& v : m->H) release(v);
     for (auto& v : m->Z) release(v);
     release(m->X); release(m->T); release(m->dA); release(m->dB); release(m->part); release(m->loss);
-    release(m->dsX); release(m->dsT);
+    release(m->dsX); release(m->dsT); release(m->wt); release(m->dboff);
     if (m->s) (void)hipStreamDestroy(m->s);
     delete m;
     return FIPC_OK;
@@ -421,6 +555,7 @@ extern "C" int fipc_mlp_set_params(fipc_mlp* m, const float* w)
 {
     if (!m || !w) return FIPC_ERR_ARG;
     HIPCHK(hipMemcpyAsync(m->params.p, w, m->nparams * 4, hipMemcpyHostToDevice, m->s));
+    m->wt_ok = false;
     HIPCHK(hipMemsetAsync(m->am.p, 0, m->nparams * 4, m->s));
     HIPCHK(hipMemsetAsync(m->av.p, 0, m->nparams * 4, m->s));
     m->adam_t = 0;
@@ -434,8 +569,9 @@ extern "C" int fipc_mlp_forward(fipc_mlp* m, int n, const float* X, float* Y)
     if (!n) return FIPC_OK;
     STCHK(mlp_reserve(m, n, true));
     HIPCHK(hipMemcpyAsync(m->X.p, X, (size_t)n * m->n_in * 4, hipMemcpyHostToDevice, m->s));
-    STCHK(mlp_forward(m, m->s, n, m->X.p, false));
-    HIPCHK(hipMemcpyAsync(Y, m->H[m->L].p, (size_t)n * m->n_out * 4, hipMemcpyDeviceToHost, m->s));
+    const float* Yd = nullptr;
+    STCHK(mlp_infer(m, m->s, n, m->X.p, &Yd));
+    HIPCHK(hipMemcpyAsync(Y, Yd, (size_t)n * m->n_out * 4, hipMemcpyDeviceToHost, m->s));
     HIPCHK(hipStreamSynchronize(m->s));
     return FIPC_OK;
 }
@@ -515,6 +651,7 @@ extern "C" int fipc_mlp_train(fipc_mlp* m, int iters, int batch, float lr, unsig
         float c1 = 1.f - powf(b1, (float)m->adam_t), c2 = 1.f - powf(b2, (float)m->adam_t);
         k_adam<<<nblk(m->nparams), TPB, 0, m->s>>>(m->nparams, m->params.p, m->grads.p, m->am.p, m->av.p, lr, b1, b2,
                                                     eps, c1, c2);
+        m->wt_ok = false;
         HIPCHK(hipGetLastError());
     }
     std::vector<float> lp((size_t)iters * nb);

@@ -25,8 +25,11 @@ struct RowSet {
 // per vertex: the smallest (squared distance, row) key over the contact rows
 // it belongs to.  Key = float bits of d^2 (order-preserving for d^2 >= 0)
 // << 32 | kind << 28 | row.
+// (Dirichlet vertices are skipped: their start is prescribed, and the rows of
+// a moving plate would otherwise all contend for its few vertices)
 template <int KIND>
-__global__ void k_ws_keys(int n, const int* __restrict__ rows, const double* __restrict__ x, unsigned long long* keys)
+__global__ void k_ws_keys(int n, const int* __restrict__ rows, const double* __restrict__ x,
+                          const unsigned char* __restrict__ dmask, unsigned long long* keys)
 {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
@@ -37,7 +40,8 @@ __global__ void k_ws_keys(int n, const int* __restrict__ rows, const double* __r
     float d2 = (float)constraint_dist(KIND, px);
     unsigned long long key = ((unsigned long long)__float_as_uint(d2) << 32) | ((unsigned)KIND << 28) | (unsigned)r;
 #pragma unroll
-    for (int a = 0; a < K; a++) atomicMin(&keys[id[a]], key);
+    for (int a = 0; a < K; a++)
+        if (!dmask[id[a]]) atomicMin(&keys[id[a]], key);
 }
 
 // the two primitives of a row: PP {0}|{1}, PE {0}|{1,2}, PT {0}|{1,2,3},
@@ -150,7 +154,7 @@ static int ws_features(fipc_ctx* c, double scale)
         rs.n[t] = c->count[t];
         int n = c->count[t];
         if (!n) continue;
-#define LAUNCH_K(K) k_ws_keys<K><<<nblk(n), TPB, 0, c->s>>>(n, c->rows[t].p, c->x.p, c->ckey.p)
+#define LAUNCH_K(K) k_ws_keys<K><<<nblk(n), TPB, 0, c->s>>>(n, c->rows[t].p, c->x.p, c->dmask.p, c->ckey.p)
         switch (t) {
         case 0: LAUNCH_K(0); break;
         case 1: LAUNCH_K(1); break;
