@@ -86,7 +86,15 @@ _SIGS = {
     "fipc_ws_capture_end": [_VP, _VP, _D],
 }
 _SCALAR = {
+    "project_pd_2": [_D] * 4 + [_P] * 4,
     "project_pd_3": [_D] * 9 + [_P] * 9,
+    "project_pd_4": [_D] * 16 + [_P] * 16,
+    "solve_2": [_D] * 6 + [_P] * 2,
+    "solve_3": [_D] * 12 + [_P] * 3,
+    "solve_4": [_D] * 20 + [_P] * 4,
+    "solve_6": [_D] * 42 + [_P] * 6,
+    "solve_9": [_D] * 90 + [_P] * 9,
+    "svd_2": [_D] * 4 + [_P] * 12,
     "project_pd_6": [_D] * 36 + [_P] * 36,
     "project_pd_9": [_D] * 81 + [_P] * 81,
     "project_pd_12": [_D] * 144 + [_P] * 144,

@@ -51,7 +51,32 @@ void project_pd_9(double, double, double, double, double, double, double, double
                    double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*);
 void project_pd_12(double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double,
                    double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*);
-/* array forms of the same projection, for callers that are not Taichi */
+/* replaces wrapper.cpp:42-48 project_pd_2 and :58-64 project_pd_4 (JGSL::makePD) */
+void project_pd_2(double, double, double, double,
+                  double*, double*, double*, double*);
+void project_pd_4(double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double,
+                  double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*, double*);
+/* replaces wrapper.cpp:90-198 solve_2 / solve_3 / solve_4 / solve_6 / solve_9:
+ * n*n matrix entries and n right-hand-side entries by value, n outputs.  LDL^T
+ * with diagonal pivoting, D clamped from below at 1e-8, then (L D L^T) x = rhs
+ * in the pivoted order, as the reference computes it (which leaves the
+ * permutation in place: x solves the clamped P F P^T) */
+void solve_2(double, double, double, double, double, double,
+             double*, double*);
+void solve_3(double, double, double, double, double, double, double, double, double, double, double, double,
+             double*, double*, double*);
+void solve_4(double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double,
+             double*, double*, double*, double*);
+void solve_6(double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double,
+             double*, double*, double*, double*, double*, double*);
+void solve_9(double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double, double,
+             double*, double*, double*, double*, double*, double*, double*, double*, double*);
+/* replaces wrapper.cpp:12-25 svd_2 (SINGULAR_VALUE_DECOMPOSITION.h:25-105):
+ * A = U diag(s00, s11) V^T, U and V rotations, s00 >= s11 (s11 may be negative) */
+void svd_2(double F00, double F01, double F10, double F11, double* U00, double* U01, double* U10, double* U11,
+           double* s00, double* s01, double* s10, double* s11, double* V00, double* V01, double* V10, double* V11);
+/* array form of the projection for n = 3, 4, 6, 9, 12 (JGSL::makePD); n = 2 is
+ * the neo-Hookean 2x2 closed form of common/math/math_tools.py:39-64 */
 void fipc_project_pd(int n, const double* in, double* out);
 /* replaces wrapper.cpp:27-40 svd_3 (rotation-variant SVD, det U = det V = 1) */
 void svd_3(double F00, double F01, double F02, double F10, double F11, double F12, double F20, double F21, double F22,

@@ -209,10 +209,16 @@ def ref():
     global _ref
     if _ref is None and os.path.exists(ref_path()):
         r = C.CDLL(ref_path())
-        for n in (3, 6, 9, 12):
+        for n in (2, 3, 4, 6, 9, 12):
             f = getattr(r, f"project_pd_{n}")
             f.restype = None
             f.argtypes = [_D] * (n * n) + [_P] * (n * n)
+        for n in (2, 3, 4, 6, 9):
+            f = getattr(r, f"solve_{n}")
+            f.restype = None
+            f.argtypes = [_D] * (n * n + n) + [_P] * n
+        r.svd_2.restype = None
+        r.svd_2.argtypes = [_D] * 4 + [_P] * 12
         r.point_triangle_ccd.restype = None
         r.point_triangle_ccd.argtypes = [_D] * 26 + [_P]
         r.edge_edge_ccd.restype = None
@@ -229,6 +235,22 @@ def ref_project_pd(H):
     outs = [C.c_double() for _ in range(n * n)]
     getattr(ref(), f"project_pd_{n}")(*[float(v) for v in H.ravel()], *[C.byref(o) for o in outs])
     return np.array([o.value for o in outs]).reshape(n, n)
+
+
+def ref_solve(A, b):
+    A = f64(A)
+    n = A.shape[0]
+    outs = [C.c_double() for _ in range(n)]
+    getattr(ref(), f"solve_{n}")(*[float(v) for v in A.ravel()], *[float(v) for v in np.ravel(b)],
+                                  *[C.byref(o) for o in outs])
+    return np.array([o.value for o in outs])
+
+
+def ref_svd2(F):
+    outs = [C.c_double() for _ in range(12)]
+    ref().svd_2(*[float(v) for v in np.ravel(F)], *[C.byref(o) for o in outs])
+    v = np.array([o.value for o in outs])
+    return v[:4].reshape(2, 2), v[4:8].reshape(2, 2), v[8:].reshape(2, 2)
 
 
 def ref_pt_ccd(p, t0, t1, t2, dp, dt0, dt1, dt2, eta, dist2):

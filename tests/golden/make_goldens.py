@@ -8,6 +8,8 @@
   and of the neo-Hookean energy Psi(F) (common/physics/neo_hookean.py:5-16):
   inputs, values, gradients, Hessians.  The formulas here are the plain
   definitions; derivatives come from torch, independent of the oracle's jets.
+* reference_dense.npz -- solve_N, svd_2 and project_pd_{2,4} of the same
+  native library, on seeded SPD / indefinite / diagonal inputs;
 * reference.npz -- outputs of the reference's own native library
   (common/math/wrapper/wrapper.cpp + EVCTCD, built by oracle/build_ref.sh into
   oracle/_ref/libref.so): project_pd_{3,6,9,12}, the cubic root and the
@@ -199,10 +201,45 @@ def reference_set(rng):
     return out
 
 
+def dense_set(rng):
+    """solve_N, svd_2 and project_pd_{2,4} of the reference's native library."""
+    from oracle import lib as O
+    if O.ref() is None:
+        return None
+    out = {}
+    for n in (2, 3, 4, 6, 9):
+        A = rng.normal(size=(60, n, n))
+        A = A + np.transpose(A, (0, 2, 1))
+        # half of them SPD, half indefinite (the clamp of D at 1e-8 matters)
+        A[:30] = np.einsum("kij,klj->kil", A[:30], A[:30]) + 0.1 * np.eye(n)
+        b = rng.normal(size=(60, n))
+        out[f"solve{n}_A"] = A
+        out[f"solve{n}_b"] = b
+        out[f"solve{n}_x"] = np.array([O.ref_solve(a, bb) for a, bb in zip(A, b)])
+    F = rng.normal(size=(200, 2, 2))
+    F[::7, 0, 1] = 0.0
+    F[::7, 1, 0] = 0.0
+    F[::11] = np.abs(F[::11]) * np.array([[1, 0], [0, 1]])
+    out["svd2_F"] = F
+    r = [O.ref_svd2(f) for f in F]
+    out["svd2_U"] = np.array([u for u, _, _ in r])
+    out["svd2_S"] = np.array([s for _, s, _ in r])
+    out["svd2_V"] = np.array([v for _, _, v in r])
+    for n in (2, 4):
+        A = rng.normal(size=(60, n, n))
+        A = A + np.transpose(A, (0, 2, 1))
+        out[f"pd{n}_in"] = A
+        out[f"pd{n}_out"] = np.array([O.ref_project_pd(a) for a in A])
+    return out
+
+
 if __name__ == "__main__":
     rng = np.random.default_rng(20261202)
     np.savez_compressed(os.path.join(HERE, "autodiff.npz"), **autodiff_set(rng))
     ref = reference_set(np.random.default_rng(424242))
     if ref is not None:
         np.savez_compressed(os.path.join(HERE, "reference.npz"), **ref)
+    dense = dense_set(np.random.default_rng(777))
+    if dense is not None:
+        np.savez_compressed(os.path.join(HERE, "reference_dense.npz"), **dense)
     print("wrote goldens")

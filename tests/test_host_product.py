@@ -148,3 +148,60 @@ def test_host_tet_math_vs_oracle():
                      O.dptr(g12), O.dptr(H12))
         assert np.abs(H - H12).max() <= 1e-9 * np.abs(H12).max()
         assert np.abs(P.reshape(3, 3) - O.nh_P(F, la, mu)).max() <= 1e-10 * np.abs(P).max()
+
+
+def _dense():
+    return np.load(os.path.join(ROOT, "tests", "golden", "reference_dense.npz"))
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 6, 9])
+def test_solve_dropins_vs_reference(n):
+    """solve_N against the reference's own compiled wrapper.cpp (goldens):
+    SPD inputs to rounding; indefinite inputs (D clamped at 1e-8, so the
+    solved matrix has a condition number near 1e8 |D|) to that conditioning.
+    The reference keeps LDL^T's pivot permutation in the solved matrix, so
+    for SPD inputs x differs from F^-1 rhs whenever the pivoting reorders."""
+    d = _dense()
+    so = _lib.load()
+    A, B, X = d[f"solve{n}_A"], d[f"solve{n}_b"], d[f"solve{n}_x"]
+    differs = 0
+    for k, (a, b, x) in enumerate(zip(A, B, X)):
+        outs = [C.c_double() for _ in range(n)]
+        getattr(so, f"solve_{n}")(*[float(v) for v in a.ravel()], *[float(v) for v in b], *[C.byref(o) for o in outs])
+        got = np.array([o.value for o in outs])
+        rel = np.abs(got - x).max() / np.abs(x).max()
+        if k < 30:  # SPD half
+            assert rel <= 1e-12, (k, rel)
+            differs += np.abs(np.linalg.solve(a, b) - x).max() > 1e-6 * np.abs(x).max()
+        else:
+            assert rel <= 1e-2, (k, rel)
+    if n >= 3:
+        assert differs > 0  # the permutation quirk is exercised
+
+
+def test_svd2_dropin_vs_reference():
+    d = _dense()
+    so = _lib.load()
+    for F, U, S, V in zip(d["svd2_F"], d["svd2_U"], d["svd2_S"], d["svd2_V"]):
+        outs = [C.c_double() for _ in range(12)]
+        so.svd_2(*[float(v) for v in F.ravel()], *[C.byref(o) for o in outs])
+        v = np.array([o.value for o in outs])
+        u, s, w = v[:4].reshape(2, 2), v[4:8].reshape(2, 2), v[8:].reshape(2, 2)
+        assert np.abs(u - U).max() <= 1e-14 and np.abs(s - S).max() <= 1e-13 and np.abs(w - V).max() <= 1e-14
+        # and what the mathematics fixes: A = U S V^T with rotations, sorted
+        assert np.abs(u @ s @ w.T - F).max() <= 1e-13 * max(1.0, np.abs(F).max())
+        assert np.linalg.det(u) == pytest.approx(1.0) and np.linalg.det(w) == pytest.approx(1.0)
+        assert s[0, 0] >= s[1, 1] and s[0, 0] >= abs(s[1, 1]) - 1e-12
+
+
+@pytest.mark.parametrize("n", [2, 4])
+def test_project_pd_2_4_dropins_vs_reference(n):
+    d = _dense()
+    so = _lib.load()
+    for a, o in zip(d[f"pd{n}_in"], d[f"pd{n}_out"]):
+        outs = [C.c_double() for _ in range(n * n)]
+        getattr(so, f"project_pd_{n}")(*[float(v) for v in a.ravel()], *[C.byref(x) for x in outs])
+        got = np.array([x.value for x in outs]).reshape(n, n)
+        assert np.abs(got - o).max() <= 1e-12 * np.abs(a).max()
+        # PSD and equal to the eigen-clamp of the leading negative run (eigh is unique for it)
+        assert np.linalg.eigvalsh(got).min() >= -1e-12 * np.abs(a).max()
