@@ -1,6 +1,13 @@
 """Benchmark: simulation steps/s of the IPC Newton step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain|fp32]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--mode chosen|plain|fp32|fp32+extrapolation] [--all-variants]
+
+The value is the steps/s of --mode (default "chosen": the learned Newton warm
+start with the FP32 Hessian and FP64 residual refinement); the plain FP64
+method (x = x_n start, FP64 Hessian) is timed in the same run for the ratio.
+Every variant uses the same Newton tolerance and checks every step for
+intersections.
 
 Workload: the compressed slab, a 50 x 80 x 50-vertex lattice (200,000
 vertices, 948,395 tets, E = 1e6, nu = 0.45) between two Dirichlet plates that
@@ -76,20 +83,45 @@ def algorithmic_bytes(sim, precision):
     }
 
 
-def run_gpu(args):
+# the variants of the step: (FP32 Hessian, warm start: None | "learned" | "extrapolation")
+VARIANTS = {
+    "plain": (False, None),               # the reference's method on the GPU: FP64, x = x_n start
+    "fp32": (True, None),
+    "fp32+extrapolation": (True, "extrapolation"),
+    "chosen": (True, "learned"),          # learned warm start + FP32 Hessian
+}
+WEIGHTS = os.path.join(ROOT, "fastipc_amd", "weights", "ws_slab.npz")
+
+
+def run_gpu(args, variant):
     from fastipc_amd import _lib
     from fastipc_amd import scene as S
     from fastipc_amd.newton import NewtonIPC
+    from fastipc_amd import warmstart as W
 
     sc = S.compressed_slab(50, 80, 50, steps=args.warmup + args.steps + args.roofline_steps)
-    precision = 1 if args.mode == "fp32" else 0
+    fp32, ws = VARIANTS[variant]
+    precision = 1 if fp32 else 0
     t0 = time.perf_counter()
     sim = NewtonIPC(sc, precision=precision, pcg_tol=args.pcg_tol)
+    hook, net, warm_meta = None, None, None
+    if ws == "learned":
+        if not os.path.exists(WEIGHTS):
+            raise FileNotFoundError(f"{WEIGHTS} is missing (python -m fastipc_amd.warmstart train ...)")
+        net = W.WarmStartMLP.load(WEIGHTS)
+        warm_meta = {k: net.meta.get(k) for k in ("family", "size", "seeds", "steps", "iters", "rows")}
+        hook = W.LearnedWarmStart(net, W.scale_for(sc))
+    elif ws == "extrapolation":
+        # the zero network: the start repeats the previous step's correction
+        net = W.WarmStartMLP(seed=0)
+        net.set_params(np.zeros(net.nparams, np.float32))
+        hook = W.LearnedWarmStart(net, W.scale_for(sc))
     setup_s = time.perf_counter() - t0
     newton = []
+    accepted = 0
     for i in range(args.warmup):
         t1 = time.perf_counter()
-        st = sim.step()
+        st = sim.step(warm_start=hook)
         log(f"warmup step {i}: {time.perf_counter() - t1:.3f}s newton {st.newton_iters} pcg {st.pcg_iters} "
             f"contacts {st.counts[-1] if st.counts else None}")
     _lib.call("fipc_sync", sim.ctx)
@@ -97,8 +129,10 @@ def run_gpu(args):
     pcg = []
     for i in range(args.steps):
         t1 = time.perf_counter()
-        st = sim.step()
-        log(f"timed step {i}: {time.perf_counter() - t1:.3f}s newton {st.newton_iters} pcg {st.pcg_iters}")
+        st = sim.step(warm_start=hook)
+        log(f"[{variant}] timed step {i}: {time.perf_counter() - t1:.3f}s newton {st.newton_iters} "
+            f"pcg {st.pcg_iters} warm start {'kept' if st.warm_accepted else '-'}")
+        accepted += int(st.warm_accepted)
         newton.append(st.newton_iters)
         pcg.extend(st.pcg_iters)
         if sim.check_collision():
@@ -118,7 +152,7 @@ def run_gpu(args):
     # the event records do not perturb the steps/s above)
     _lib.call("fipc_enable_timing", sim.ctx, 1)
     for i in range(args.roofline_steps):
-        st = sim.step()
+        st = sim.step(warm_start=hook)
         log(f"event-timed step {i}: newton {st.newton_iters} pcg {st.pcg_iters}")
         if sim.check_collision():
             raise RuntimeError("intersection after a step")
@@ -142,11 +176,15 @@ def run_gpu(args):
     achieved = nbytes[dom] / avg_s / 1e9 if avg_s > 0 else 0.0
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(args.mode) if dom == "spmv" else None, "bytes_per_launch": nbytes[dom],
+            "traffic": measured_traffic("fp32" if precision == 1 else "plain") if dom == "spmv" else None,
+            "bytes_per_launch": nbytes[dom],
             "avg_launch_us": round(kt[dom]["us_avg"], 2), "launches": kt[dom]["launches"]}
     others = {k: {"achieved_GBs": round(nbytes[k] / (kt[k]["us_avg"] * 1e-6) / 1e9, 1) if kt[k]["us_avg"] else None,
                   "avg_launch_us": round(kt[k]["us_avg"], 2)} for k in nbytes}
     res = {
+        "variant": variant,
+        "warm_start_kept": accepted,
+        "warm_start_weights": warm_meta,
         "steps_per_s": args.steps / wall,
         "ms_per_step": wall / args.steps * 1e3,
         "newton_iters_per_step": float(np.mean(newton)),
@@ -160,6 +198,8 @@ def run_gpu(args):
         "nt": len(sc.tets),
     }
     sim.close()
+    if net is not None:
+        net.close()
     return res
 
 
@@ -200,7 +240,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=14)
-    ap.add_argument("--mode", choices=("plain", "fp32"), default="plain")
+    ap.add_argument("--mode", choices=tuple(VARIANTS), default="chosen",
+                    help="the variant whose steps/s is the value (default: learned warm start + FP32)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run timed for the ratio")
+    ap.add_argument("--all-variants", action="store_true", help="also time fp32 and fp32+extrapolation")
     ap.add_argument("--pcg-tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
@@ -216,10 +259,21 @@ def main():
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design note, not an implementation)", file=sys.stderr)
         sys.exit(2)
-    g = run_gpu(args)
+    g = run_gpu(args, args.mode)
     if g is None:
         return
+    others = {}
+    extra = [] if args.no_plain or args.mode == "plain" else ["plain"]
+    if args.all_variants:
+        extra += [v for v in VARIANTS if v not in (args.mode, "plain")]
+    for v in extra:
+        r = run_gpu(args, v)
+        others[v] = {"steps_per_s": round(r["steps_per_s"], 4), "ms_per_step": round(r["ms_per_step"], 3),
+                     "newton_iters_per_step": round(r["newton_iters_per_step"], 2),
+                     "pcg_iters_per_newton": round(r["pcg_iters_per_newton"], 1),
+                     "warm_start_kept": r["warm_start_kept"]}
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(args)
+    fp32, ws = VARIANTS[args.mode]
     line = {
         "metric": "sim steps/s at 1M tets, warm-start+FP32 vs plain FP64 GPU at equal residual",
         "value": round(g["steps_per_s"], 4),
@@ -231,13 +285,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.mode == "plain" else "f32-hessian+f64",
+        "dtype": "f32-hessian+f64" if fp32 else "f64",
         "data": "synthetic (seeded scene generator)",
         "config": {"workload": "compressed_slab 50x80x50 (200,000 verts, 948,395 tets), plates closing 0.2 H/s, "
-                               "twist 30 deg, dt 0.005", "mode": args.mode, "nv": g["nv"], "nt": g["nt"],
+                               "twist 30 deg, dt 0.005", "variant": args.mode, "warm_start": ws,
+                   "warm_start_weights": g["warm_start_weights"], "nv": g["nv"], "nt": g["nt"],
                    "newton_iters_per_step": round(g["newton_iters_per_step"], 2),
                    "pcg_iters_per_newton": round(g["pcg_iters_per_newton"], 1),
+                   "warm_start_kept": g["warm_start_kept"],
+                   "newton_tol": "1e-2 dt bbox on max|p| (every variant)", "intersections": 0,
                    "pcg_rel_tol": args.pcg_tol, "contacts_last_step": g["contacts_last_step"]},
+        "same_run_variants": others,
+        "speedup_vs_plain_fp64": round(g["steps_per_s"] / others["plain"]["steps_per_s"], 4) if "plain" in others else None,
         "roofline": g["roofline"],
         "cpu_baseline": cpu,
         "kernel_classes": g["kernel_classes"],

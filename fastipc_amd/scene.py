@@ -270,9 +270,13 @@ def cube_grid(n: int = 16, grid: int = 3, seed: int = 0, steps: int = 100) -> Sc
     L = grid * spacing
     th = 0.1 * edge
     H = L + spacing
+    # ground and four walls as separate boxes, kept apart by gaps well above
+    # dHat: IPC needs an intersection-free start, and touching static boxes
+    # would put zero-distance pairs between fixed vertices
+    gap = 0.05 * edge
     for lo, hi in [([-th, -th, -th], [L + th, 0.0, L + th]),
-                   ([-th, 0.0, -th], [0.0, H, L + th]), ([L, 0.0, -th], [L + th, H, L + th]),
-                   ([0.0, 0.0, -th], [L, H, 0.0]), ([0.0, 0.0, L], [L, H, L + th])]:
+                   ([-th, gap, -th], [0.0, H, L + th]), ([L, gap, -th], [L + th, H, L + th]),
+                   ([gap, gap, -th], [L - gap, H, 0.0]), ([gap, gap, L], [L - gap, H, L + th])]:
         gx, gt = box(lo, hi)
         b.add(gx, gt, fixed=True)
     return b.build(name=f"cube_grid_{grid}x{n}", dt=0.01, steps=steps, seed=seed)

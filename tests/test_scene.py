@@ -92,3 +92,15 @@ def test_seeded_scenes_are_reproducible():
     b = S.cube_grid(4, 2, seed=3)
     c = S.cube_grid(4, 2, seed=4)
     assert np.array_equal(a.x, b.x) and not np.array_equal(a.x, c.x)
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 4])
+def test_config_initial_states_are_intersection_free(cfg):
+    """IPC needs an intersection-free start: the oracle's collision check finds
+    no edge crossing a triangle and no pair is within dHat (in particular no
+    pair of fixed vertices: static boxes do not touch)."""
+    from oracle.newton import OracleIPC
+    sc = S.config(cfg)
+    o = OracleIPC(sc)
+    assert o.check_collision() == 0
+    assert sum(o.find_constraints()) == 0
