@@ -66,20 +66,23 @@ def algorithmic_bytes(sim, precision):
     """Compulsory HBM bytes per launch of the kernel classes."""
     s = sim.setup
     nv, nt = sim.nv, len(s.tets)
-    nnzb = C.c_int()
+    nu, nl = C.c_int(), C.c_int()
     from fastipc_amd import _lib
-    _lib.call("fipc_get_bsr", sim.ctx, C.byref(nnzb), None, None, None)
-    nnzb = nnzb.value
+    _lib.call("fipc_bsr_storage", sim.ctx, C.byref(nu), C.byref(nl))
+    nu, nl = nu.value, nl.value
     sa = 4 if precision == 1 else 8
     width = (2, 3, 4, 4, 4, 4, 4)
     contact = sum(n * ((3 * w) ** 2 * 8 + 4 * w) for n, w in zip(sim.counts, width))
     return {
-        # BSR values + column index per block, row pointers, p read, q written,
-        # contact rows (dense H_r + indices)
-        "spmv": nnzb * (9 * sa + 4) + 4 * (nv + 1) + 2 * 3 * nv * 8 + contact,
-        # per tet: 4 indices, D_m^-1, vol, lambda, mu, 16 slots; per vertex x;
-        # the BSR values written once and the gradient
-        "elastic_assembly": nt * (16 + 72 + 24 + 64) + nv * 24 + nnzb * 9 * 8 + 3 * nv * 8,
+        # stored (upper) block values + column index, (block, column) pair per
+        # lower block, both row-pointer arrays, p read, q written, contact rows
+        # (dense H_r + indices); the transposed re-read of a stored block is
+        # not compulsory traffic
+        "spmv": nu * (9 * sa + 4) + nl * 8 + 8 * (nv + 1) + 2 * 3 * nv * 8 + contact,
+        # per tet: 4 indices, D_m^-1, vol, lambda, mu, 10 contribution
+        # positions; per vertex x; the stored block values written once and
+        # the gradient
+        "elastic_assembly": nt * (16 + 72 + 24 + 40) + nv * 24 + nu * 9 * 8 + 3 * nv * 8,
     }
 
 

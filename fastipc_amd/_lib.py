@@ -58,6 +58,7 @@ _SIGS = {
     "fipc_get_vector": [_VP, C.c_int, _P],
     "fipc_set_vector": [_VP, C.c_int, _P],
     "fipc_get_bsr": [_VP, _IP, _IP, _IP, _P],
+    "fipc_bsr_storage": [_VP, _IP, _IP],
     "fipc_spmv": [_VP, _P, _P],
     "fipc_enable_timing": [_VP, C.c_int],
     "fipc_kernel_times": [_VP, C.c_int, _P, C.POINTER(C.c_long)],

@@ -93,15 +93,22 @@ struct fipc_ctx {
     // state
     DVec<double> x, x0, xn, xt, xprev, v, sol, rhs, target;
     double dt = 0.01, dHat = 1e-3, dHat2 = 1e-6, kappa = 1.0, inv_dx = 1.0, gravity = -9.8;
-    // elastic + inertia BSR (3x3 blocks, values as 9 planes of nnzb)
-    int nnzb = 0;
-    DVec<int> rowptr, col, diag, slots, brow;
+    // elastic + inertia Hessian, symmetric: only the upper-triangle 3x3
+    // blocks are stored (row i holds columns j >= i, the diagonal block
+    // first), 9 values per block row-major (val[9 b + 3 r + c]); for every
+    // row i, lent[lptr[i] .. lptr[i + 1]) lists the pairs (block, j) of the
+    // blocks (j, i), j < i, stored in row j, which the SpMV applies
+    // transposed.  nnzb_full counts both triangles.
+    int nnzb = 0, nnzb_full = 0, nlower = 0;
+    DVec<int> rowptr, col, diag, brow, lptr, lent;
     DVec<double> val;
     DVec<float> valf;
-    // deterministic assembly: per-tet upper blocks and gradients, gathered
-    // through block -> (slot, tet) and vertex -> (corner, tet) lists
+    // assembly: the per-tet pass writes each of a tet's 10 upper blocks
+    // (oriented to its global block) to cpos[10 e + slot] of a scratch sorted
+    // by block, so the gather sums contiguous runs bcptr[b] .. bcptr[b + 1];
+    // the gradient is gathered through vertex -> (corner, tet) lists
     DVec<double> tetH, tetg;
-    DVec<int> bcptr, bcent, vcptr, vcent;
+    DVec<int> cpos, bcptr, vcptr, vcent;
     int precision = 0;     // 0 = FP64 Hessian, 1 = FP32 Hessian / SpMV
     // contacts
     int64_t cap[NTYPES] = {0};

@@ -3,10 +3,10 @@
 //
 // Data layout in HBM (per context):
 //   x, x0, xn, xTilde, xPrev, v, p, rhs : float64 [nv][3]
-//   elastic + inertia Hessian          : BSR, 3x3 blocks, values as nine
-//                                        planes val[k * nnzb + b] (k = 3r + c)
-//                                        so that lanes walking consecutive
-//                                        blocks load contiguous doubles
+//   elastic + inertia Hessian          : symmetric BSR, upper-triangle 3x3
+//                                        blocks only, val[9 b + 3 r + c], so a
+//                                        block read in either orientation is
+//                                        one contiguous 36 / 72 byte run
 //   contact Hessians                   : per row, the projected (3k x 3k)
 //                                        matrix row-major, k = 2, 3, 4
 //   block-Jacobi preconditioner        : float64 [nv][9] inverse blocks
@@ -168,7 +168,8 @@ __host__ __device__ constexpr int uslot(int a, int c) { return a * 4 - a * (a - 
 __global__ void __launch_bounds__(TPB) k_elastic(int nt, const int* __restrict__ tets, const double* __restrict__ IB,
                                                  const double* __restrict__ vol, const double* __restrict__ la,
                                                  const double* __restrict__ mu, const double* __restrict__ x,
-                                                 double dt2, double* __restrict__ tetH, double* __restrict__ tetg)
+                                                 double dt2, const int* __restrict__ cpos, double* __restrict__ tetH,
+                                                 double* __restrict__ tetg)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nt) return;
@@ -186,51 +187,48 @@ __global__ void __launch_bounds__(TPB) k_elastic(int nt, const int* __restrict__
             tetg[((size_t)a * nt + e) * 3 + i] = (P[i * 3] * Bm[a][0] + P[i * 3 + 1] * Bm[a][1] + P[i * 3 + 2] * Bm[a][2]) * sc;
     NHFactor f;
     nh_factor(F, lae, mue, f);
+    // block (a, c), a <= c, belongs to the upper block (min, max) of the global
+    // vertices: transposed when the global order is reversed
     tet_hessian_blocks(f, Bm, sc, [&](int a, int c, const double* blk) {
-        double* o = tetH + ((size_t)uslot(a, c) * nt + e) * 9;
+        double* o = tetH + (size_t)cpos[(size_t)e * 10 + uslot(a, c)] * 9;
+        if (t[a] <= t[c]) {
 #pragma unroll
-        for (int k = 0; k < 9; k++) o[k] = blk[k];
+            for (int k = 0; k < 9; k++) o[k] = blk[k];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) o[i * 3 + j] = blk[j * 3 + i];
+        }
     });
 }
 
-// second pass, one thread per BSR block: sum the contributing tet blocks in
-// a fixed order (transposed for the lower triangle), add the lumped mass on
-// diagonal blocks (compute_inertia, NEWTON_IPC.py:558-567), store the FP64
-// planes and, when valf != nullptr, their FP32 copy for the FP32 SpMV
-__global__ void k_gather_blocks(int nnzb, const int* __restrict__ bcptr, const int* __restrict__ bcent,
-                                const double* __restrict__ tetH, const int* __restrict__ brow,
-                                const int* __restrict__ diag, const double* __restrict__ m, double* __restrict__ val,
-                                float* __restrict__ valf)
+// second pass, one thread per stored (upper) block: sum its contiguous run of
+// oriented tet blocks in the scratch (tets in index order), add the lumped
+// mass on diagonal blocks (compute_inertia, NEWTON_IPC.py:558-567), store the
+// FP64 planes and, when valf != nullptr, their FP32 copy for the FP32 SpMV
+__global__ void k_gather_blocks(int nnzb, const int* __restrict__ bcptr, const double* __restrict__ tetH,
+                                const int* __restrict__ brow, const int* __restrict__ col,
+                                const double* __restrict__ m, double* __restrict__ val, float* __restrict__ valf)
 {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nnzb) return;
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int t = bcptr[b]; t < bcptr[b + 1]; t++) {
-        unsigned en = (unsigned)bcent[t];
-        const double* h = tetH + (size_t)(en & 0x7fffffffu) * 9;
-        double v[9];
+        const double* h = tetH + (size_t)t * 9;
 #pragma unroll
-        for (int k = 0; k < 9; k++) v[k] = h[k];
-        if (en >> 31) {
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) acc[i * 3 + j] += v[j * 3 + i];
-        } else {
-#pragma unroll
-            for (int k = 0; k < 9; k++) acc[k] += v[k];
-        }
+        for (int k = 0; k < 9; k++) acc[k] += h[k];
     }
     int r = brow[b];
-    if (diag[r] == b) {
+    if (col[b] == r) {
         double mi = m[r];
         acc[0] += mi; acc[4] += mi; acc[8] += mi;
     }
 #pragma unroll
-    for (int k = 0; k < 9; k++) val[(size_t)k * nnzb + b] = acc[k];
+    for (int k = 0; k < 9; k++) val[(size_t)b * 9 + k] = acc[k];
     if (valf) {
 #pragma unroll
-        for (int k = 0; k < 9; k++) valf[(size_t)k * nnzb + b] = (float)acc[k];
+        for (int k = 0; k < 9; k++) valf[(size_t)b * 9 + k] = (float)acc[k];
     }
 }
 
@@ -352,14 +350,14 @@ __global__ void __launch_bounds__(TPB) k_contact(int n, const int* __restrict__ 
 
 // block-Jacobi preconditioner: inverse of (elastic diag block + contact diag
 // block); Dirichlet vertices get the identity (their residual is zero)
-__global__ void k_precond(int nv, int nnzb, const double* __restrict__ val, const int* __restrict__ diag,
+__global__ void k_precond(int nv, const double* __restrict__ val, const int* __restrict__ diag,
                           const double* __restrict__ cdiag, const unsigned char* __restrict__ dmask, double* Minv)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nv) return;
     double A[9];
     int b = diag[i];
-    for (int k = 0; k < 9; k++) A[k] = val[(size_t)k * nnzb + b] + cdiag[9 * i + k];
+    for (int k = 0; k < 9; k++) A[k] = val[(size_t)b * 9 + k] + cdiag[9 * i + k];
     double* o = Minv + 9 * i;
     double d = det3(A);
     if (dmask[i] || !(fabs(d) > 0.0)) {
@@ -380,9 +378,9 @@ __global__ void k_precond(int nv, int nnzb, const double* __restrict__ val, cons
 // type (first, so that they overlap the BSR rows instead of trailing them),
 // the nb_bsr blocks after them the BSR rows (nb_c is a multiple of 8, so the
 // XCD of a BSR block is that of its own index).
-//  - BSR rows: 8 lanes per block row stride over the row's blocks and reduce
-//    with shuffles; a block handles 32 consecutive rows per pass over a
-//    contiguous range of row groups (xcd_block).  q[row] = (A_bsr p)[row] is
+//  - BSR rows: 8 lanes per block row stride over the row's entries (stored
+//    blocks, then transposed lower blocks) and reduce with shuffles; a block
+//    handles 32 consecutive rows per pass.  q[row] = (A_bsr p)[row] is
 //    stored (zero on Dirichlet rows), so q needs no clearing.
 //  - contact rows: a lane group per row, one lane per vertex; the lane's three
 //    rows of H_r p_r are added atomically into qc (skipping Dirichlet vertices).
@@ -436,36 +434,71 @@ __device__ __forceinline__ double contact_rows_spmv(int n, const int* __restrict
     return pq;
 }
 template <class T>
-__global__ void __launch_bounds__(TPB) k_spmv(int nv, int nnzb, const int* __restrict__ rowptr,
-                                              const int* __restrict__ col, const T* __restrict__ val,
-                                              const double* __restrict__ p, double* __restrict__ q, double* qc,
+__global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ rowptr, const int* __restrict__ col,
+                                              const int* __restrict__ lptr, const int2* __restrict__ lent,
+                                              const T* __restrict__ val, const double* __restrict__ p,
+                                              double* __restrict__ q, double* qc,
                                               const unsigned char* __restrict__ dmask, ContactSet cs, int nb_c,
                                               int nb_bsr, double* part, const int* __restrict__ state, int it)
 {
     if (state && it > state[0]) return;
     double pq = 0.0;
     if ((int)blockIdx.x >= nb_c) {
-        int lb = xcd_block((int)blockIdx.x - nb_c, nb_bsr);
+        // the row groups are split into 8 contiguous ranges, one per XCD
+        // (blocks are dispatched to XCDs round-robin); the XCD's blocks
+        // interleave over its range, so at any time they cover consecutive
+        // rows and the transposed blocks a row needs (stored up to a few
+        // thousand rows above it) were read into the XCD's L2 by the sweep
+        // just before
+        int bb = (int)blockIdx.x - nb_c;
+        int nbx = nb_bsr >> 3;
         int groups = (nv + SPMV_ROWS - 1) / SPMV_ROWS;
-        int per = (groups + nb_bsr - 1) / nb_bsr;
-        int g0 = lb * per, g1 = min(g0 + per, groups);
+        int gpx = (groups + 7) >> 3;
+        int g1 = min((bb & 7) * gpx + gpx, groups);
+        int g0 = (bb & 7) * gpx + (bb >> 3);
         int lane = threadIdx.x % SPMV_G;
         int rsub = threadIdx.x / SPMV_G;
         // row bounds of the next group are loaded while the current one runs
         int row = g0 * SPMV_ROWS + rsub;
-        int nb0 = row < nv && g0 < g1 ? rowptr[row] : 0, nb1 = row < nv && g0 < g1 ? rowptr[row + 1] : 0;
-        for (int g = g0; g < g1; g++) {
+        bool live = row < nv && g0 < g1;
+        int nu0 = live ? rowptr[row] : 0, nu1 = live ? rowptr[row + 1] : 0;
+        int nl0 = live ? lptr[row] : 0, nl1 = live ? lptr[row + 1] : 0;
+        for (int g = g0; g < g1; g += nbx) {
             row = g * SPMV_ROWS + rsub;
-            int b0 = nb0, b1 = nb1;
-            int nrow = row + SPMV_ROWS;
-            if (g + 1 < g1 && nrow < nv) { nb0 = rowptr[nrow]; nb1 = rowptr[nrow + 1]; }
+            int u0 = nu0, l0 = nl0;
+            int nu = nu1 - nu0, n = nu + (nl1 - nl0);
+            int nrow = row + nbx * SPMV_ROWS;
+            if (g + nbx < g1 && nrow < nv) {
+                nu0 = rowptr[nrow]; nu1 = rowptr[nrow + 1];
+                nl0 = lptr[nrow]; nl1 = lptr[nrow + 1];
+            }
+            // the row's stored blocks (i, j), j >= i, then the transposes of
+            // the blocks (j, i), j < i, stored in the rows above
             double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-            for (int b = b0 + lane; b < b1; b += SPMV_G) {
-                int c = col[b];
+            for (int k = lane; k < n; k += SPMV_G) {
+                bool tr = k >= nu;
+                int b, c;
+                if (!tr) {
+                    b = u0 + k;
+                    c = col[b];
+                } else {
+                    int2 e = lent[l0 + k - nu];
+                    b = e.x;
+                    c = e.y;
+                }
+                const T* A = val + (size_t)b * 9;
+                double v0 = A[0], v1 = A[1], v2 = A[2], v3 = A[3], v4 = A[4], v5 = A[5], v6 = A[6], v7 = A[7],
+                       v8 = A[8];
+                if (tr) {
+                    double t;
+                    t = v1; v1 = v3; v3 = t;
+                    t = v2; v2 = v6; v6 = t;
+                    t = v5; v5 = v7; v7 = t;
+                }
                 double x0 = p[3 * c], x1 = p[3 * c + 1], x2 = p[3 * c + 2];
-                a0 += (double)val[0 * nnzb + b] * x0 + (double)val[1 * nnzb + b] * x1 + (double)val[2 * nnzb + b] * x2;
-                a1 += (double)val[3 * nnzb + b] * x0 + (double)val[4 * nnzb + b] * x1 + (double)val[5 * nnzb + b] * x2;
-                a2 += (double)val[6 * nnzb + b] * x0 + (double)val[7 * nnzb + b] * x1 + (double)val[8 * nnzb + b] * x2;
+                a0 += v0 * x0 + v1 * x1 + v2 * x2;
+                a1 += v3 * x0 + v4 * x1 + v5 * x2;
+                a2 += v6 * x0 + v7 * x1 + v8 * x2;
             }
 #pragma unroll
             for (int o = SPMV_G / 2; o > 0; o >>= 1) {
@@ -795,8 +828,9 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->IB); release(c->vol); release(c->la); release(c->mu); release(c->m); release(c->dmask);
     release(c->x); release(c->x0); release(c->xn); release(c->xt); release(c->xprev); release(c->v);
     release(c->sol); release(c->rhs); release(c->target);
-    release(c->rowptr); release(c->col); release(c->diag); release(c->slots); release(c->val); release(c->valf);
-    release(c->brow); release(c->tetH); release(c->tetg); release(c->bcptr); release(c->bcent); release(c->vcptr);
+    release(c->rowptr); release(c->col); release(c->diag); release(c->val); release(c->valf);
+    release(c->brow); release(c->tetH); release(c->tetg); release(c->bcptr); release(c->cpos); release(c->vcptr);
+    release(c->lptr); release(c->lent);
     release(c->vcent);
     for (int t = 0; t < NTYPES; t++) { release(c->rows[t]); release(c->cH[t]); }
     release(c->keys_pp); release(c->keys_pe); release(c->keys_tmp); release(c->counters); release(c->cub_tmp);
@@ -815,75 +849,88 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     return FIPC_OK;
 }
 
-// BSR pattern of the elastic + inertia Hessian: one block per vertex pair
-// sharing a tet (and every diagonal block).  slots[e*16 + a*4 + c] is the
-// block of (tet vertex a, tet vertex c).
+// symmetric BSR pattern of the elastic + inertia Hessian: one upper block per
+// vertex pair sharing a tet (and every diagonal block), the lower lists of the
+// SpMV, and the block-sorted positions of the per-tet contributions
 static int build_pattern(fipc_ctx* c, int nv, int nt, const int* tets)
 {
+    // upper pattern: row i holds i and its neighbours j > i, sorted
     std::vector<std::vector<int>> adj(nv);
     for (int i = 0; i < nv; i++) adj[i].push_back(i);
     for (int e = 0; e < nt; e++)
         for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++) adj[tets[4 * e + a]].push_back(tets[4 * e + b]);
+            for (int b = 0; b < 4; b++) {
+                int i = tets[4 * e + a], j = tets[4 * e + b];
+                if (j > i) adj[i].push_back(j);
+            }
     std::vector<int> rowptr(nv + 1, 0), diag(nv);
     for (int i = 0; i < nv; i++) {
         auto& v = adj[i];
         std::sort(v.begin(), v.end());
         v.erase(std::unique(v.begin(), v.end()), v.end());
         rowptr[i + 1] = rowptr[i] + (int)v.size();
+        diag[i] = rowptr[i];
     }
     int nnzb = rowptr[nv];
-    std::vector<int> col(nnzb);
+    std::vector<int> col(nnzb), brow(nnzb);
     for (int i = 0; i < nv; i++) {
         std::copy(adj[i].begin(), adj[i].end(), col.begin() + rowptr[i]);
-        diag[i] = rowptr[i] + (int)(std::lower_bound(adj[i].begin(), adj[i].end(), i) - adj[i].begin());
+        for (int b = rowptr[i]; b < rowptr[i + 1]; b++) brow[b] = i;
     }
-    std::vector<int> slots((size_t)nt * 16);
-    for (int e = 0; e < nt; e++)
-        for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++) {
-                int i = tets[4 * e + a], j = tets[4 * e + b];
-                slots[(size_t)e * 16 + a * 4 + b] = rowptr[i] + (int)(std::lower_bound(adj[i].begin(), adj[i].end(), j) - adj[i].begin());
-            }
-    // block -> contributing (slot, tet) entries, tets in index order; bit 31
-    // marks a transposed (lower-triangle) use of the slot
+    auto block_of = [&](int i, int j) {  // i <= j
+        return rowptr[i] + (int)(std::lower_bound(adj[i].begin(), adj[i].end(), j) - adj[i].begin());
+    };
+    // lower lists: for row j, the blocks (i, j), i < j, in order of i
+    std::vector<int> lptr(nv + 1, 0);
+    for (int b = 0; b < nnzb; b++)
+        if (col[b] > brow[b]) lptr[col[b] + 1]++;
+    for (int i = 0; i < nv; i++) lptr[i + 1] += lptr[i];
+    int nlower = lptr[nv];
+    std::vector<int> lent(2 * (size_t)std::max(nlower, 1)), lfill(lptr.begin(), lptr.end() - 1);
+    for (int b = 0; b < nnzb; b++)
+        if (col[b] > brow[b]) {
+            int k = lfill[col[b]]++;
+            lent[2 * k] = b;
+            lent[2 * k + 1] = brow[b];
+        }
+    // contributions: the 10 upper slots of every tet, sorted by their block
+    // (counting sort; tets in index order within a block)
+    std::vector<int> cblock((size_t)nt * 10);
     std::vector<int> bcnt(nnzb + 1, 0);
     for (int e = 0; e < nt; e++)
         for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++) bcnt[slots[(size_t)e * 16 + a * 4 + b] + 1]++;
-    for (int b = 0; b < nnzb; b++) bcnt[b + 1] += bcnt[b];
-    std::vector<int> bcent((size_t)nt * 16), fill(bcnt.begin(), bcnt.end() - 1);
-    for (int e = 0; e < nt; e++)
-        for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++) {
-                int lo = std::min(a, b), hi = std::max(a, b);
-                unsigned en = (unsigned)(uslot(lo, hi) * nt + e);
-                if (a > b) en |= 0x80000000u;
-                bcent[fill[slots[(size_t)e * 16 + a * 4 + b]]++] = (int)en;
+            for (int cc = a; cc < 4; cc++) {
+                int i = tets[4 * e + a], j = tets[4 * e + cc];
+                int bu = block_of(std::min(i, j), std::max(i, j));
+                cblock[(size_t)e * 10 + uslot(a, cc)] = bu;
+                bcnt[bu + 1]++;
             }
-    std::vector<int> brow(nnzb);
-    for (int i = 0; i < nv; i++)
-        for (int b = rowptr[i]; b < rowptr[i + 1]; b++) brow[b] = i;
-    // vertex -> (corner, tet) entries
+    for (int b = 0; b < nnzb; b++) bcnt[b + 1] += bcnt[b];
+    std::vector<int> cpos((size_t)std::max(nt, 1) * 10), fill(bcnt.begin(), bcnt.end() - 1);
+    for (size_t k = 0; k < (size_t)nt * 10; k++) cpos[k] = fill[cblock[k]]++;
+    // vertex -> (corner, tet) entries for the gradient
     std::vector<int> vcnt(nv + 1, 0);
     for (int e = 0; e < nt; e++)
         for (int a = 0; a < 4; a++) vcnt[tets[4 * e + a] + 1]++;
     for (int i = 0; i < nv; i++) vcnt[i + 1] += vcnt[i];
-    std::vector<int> vcent((size_t)nt * 4), vfill(vcnt.begin(), vcnt.end() - 1);
+    std::vector<int> vcent((size_t)std::max(nt, 1) * 4), vfill(vcnt.begin(), vcnt.end() - 1);
     for (int e = 0; e < nt; e++)
         for (int a = 0; a < 4; a++) vcent[vfill[tets[4 * e + a]]++] = a * nt + e;
     STCHK(upload(c->bcptr, bcnt.data(), bcnt.size(), c->s));
-    STCHK(upload(c->bcent, bcent.data(), bcent.size(), c->s));
+    STCHK(upload(c->cpos, cpos.data(), cpos.size(), c->s));
     STCHK(upload(c->brow, brow.data(), brow.size(), c->s));
+    STCHK(upload(c->lptr, lptr.data(), lptr.size(), c->s));
+    STCHK(upload(c->lent, lent.data(), lent.size(), c->s));
     STCHK(upload(c->vcptr, vcnt.data(), vcnt.size(), c->s));
     STCHK(upload(c->vcent, vcent.data(), vcent.size(), c->s));
     STCHK(ensure(c->tetH, (size_t)std::max(nt, 1) * 90));
     STCHK(ensure(c->tetg, (size_t)std::max(nt, 1) * 12));
     c->nnzb = nnzb;
+    c->nlower = nlower;
+    c->nnzb_full = nnzb + nlower;
     STCHK(upload(c->rowptr, rowptr.data(), rowptr.size(), c->s));
     STCHK(upload(c->col, col.data(), col.size(), c->s));
     STCHK(upload(c->diag, diag.data(), diag.size(), c->s));
-    STCHK(upload(c->slots, slots.data(), slots.size(), c->s));
     STCHK(ensure(c->val, (size_t)nnzb * 9));
     HIPCHK(hipStreamSynchronize(c->s));
     return FIPC_OK;
@@ -1079,13 +1126,13 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     STCHK(fipc_tm_begin(c, TM_ELASTIC));
     if (c->nt)
         k_elastic<<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p, c->x.p,
-                                                  c->dt * c->dt, c->tetH.p, c->tetg.p);
+                                                  c->dt * c->dt, c->cpos.p, c->tetH.p, c->tetg.p);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_ELASTIC));
     STCHK(fipc_tm_begin(c, TM_GATHER));
     if (precision == 1) STCHK(ensure(c->valf, (size_t)c->nnzb * 9));
-    k_gather_blocks<<<nblk(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->bcent.p, c->tetH.p, c->brow.p, c->diag.p,
-                                                     c->m.p, c->val.p, precision == 1 ? c->valf.p : nullptr);
+    k_gather_blocks<<<nblk(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->tetH.p, c->brow.p, c->col.p, c->m.p,
+                                                     c->val.p, precision == 1 ? c->valf.p : nullptr);
     k_gather_rhs<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->vcptr.p, c->vcent.p, c->tetg.p, c->m.p, c->x.p, c->xt.p,
                                                  c->dmask.p, c->rhs.p, c->cdiag.p);
     HIPCHK(hipGetLastError());
@@ -1093,7 +1140,7 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     STCHK(fipc_tm_begin(c, TM_CONTACT));
     STCHK(launch_contacts(c, 1));
     STCHK(fipc_tm_end(c, TM_CONTACT));
-    k_precond<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->nnzb, c->val.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p);
+    k_precond<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->val.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p);
     HIPCHK(hipGetLastError());
     return FIPC_OK;
 }
@@ -1126,11 +1173,11 @@ static int spmv(fipc_ctx* c, int prec, const double* p, double* q, double* qc, d
     const int timer = part ? TM_SPMV : TM_SPMV_RES;
     STCHK(fipc_tm_begin(c, timer));
     if (prec == 1)
-        k_spmv<float><<<nbc + nbb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->valf.p, p, q, qc, c->dmask.p,
-                                                   cs, nbc, nbb, part, state, it);
+        k_spmv<float><<<nbc + nbb, TPB, 0, c->s>>>(c->nv, c->rowptr.p, c->col.p, c->lptr.p, (const int2*)c->lent.p,
+                                                   c->valf.p, p, q, qc, c->dmask.p, cs, nbc, nbb, part, state, it);
     else
-        k_spmv<double><<<nbc + nbb, TPB, 0, c->s>>>(c->nv, c->nnzb, c->rowptr.p, c->col.p, c->val.p, p, q, qc, c->dmask.p,
-                                                    cs, nbc, nbb, part, state, it);
+        k_spmv<double><<<nbc + nbb, TPB, 0, c->s>>>(c->nv, c->rowptr.p, c->col.p, c->lptr.p, (const int2*)c->lent.p,
+                                                    c->val.p, p, q, qc, c->dmask.p, cs, nbc, nbb, part, state, it);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, timer));
     return FIPC_OK;
@@ -1371,24 +1418,53 @@ extern "C" int fipc_stream_copy(fipc_ctx* c, long nbytes, double* ms)
 extern "C" int fipc_get_bsr(fipc_ctx* c, int* nnzb, int* rowptr, int* col, double* val)
 {
     if (!c || !c->nv) return FIPC_ERR_STATE;
-    if (nnzb) *nnzb = c->nnzb;
-    if (rowptr) HIPCHK(hipMemcpyAsync(rowptr, c->rowptr.p, (c->nv + 1) * 4, hipMemcpyDeviceToHost, c->s));
-    if (col) HIPCHK(hipMemcpyAsync(col, c->col.p, (size_t)c->nnzb * 4, hipMemcpyDeviceToHost, c->s));
+    if (nnzb) *nnzb = c->nnzb_full;
+    if (!rowptr && !col && !val) return FIPC_OK;
+    // the stored upper triangle expanded to the full matrix: row i lists the
+    // transposes of (j, i), j < i, then its own blocks, columns ascending
+    int nv = c->nv, nu = c->nnzb, nl = c->nlower;
+    std::vector<int> up(nv + 1), ucol(nu), lp(nv + 1), le(2 * (size_t)std::max(nl, 1));
+    std::vector<double> v((size_t)nu * 9);
+    HIPCHK(hipMemcpyAsync(up.data(), c->rowptr.p, up.size() * 4, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(ucol.data(), c->col.p, ucol.size() * 4, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(lp.data(), c->lptr.p, lp.size() * 4, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(le.data(), c->lent.p, le.size() * 4, hipMemcpyDeviceToHost, c->s));
     if (val) {
-        std::vector<double> planes((size_t)c->nnzb * 9);
         if (c->precision == 1) {
-            std::vector<float> pf(planes.size());
-            HIPCHK(hipMemcpyAsync(pf.data(), c->valf.p, pf.size() * 4, hipMemcpyDeviceToHost, c->s));
+            std::vector<float> vf(v.size());
+            HIPCHK(hipMemcpyAsync(vf.data(), c->valf.p, vf.size() * 4, hipMemcpyDeviceToHost, c->s));
             HIPCHK(hipStreamSynchronize(c->s));
-            for (size_t i = 0; i < pf.size(); i++) planes[i] = pf[i];
+            for (size_t i = 0; i < vf.size(); i++) v[i] = vf[i];
         } else {
-            HIPCHK(hipMemcpyAsync(planes.data(), c->val.p, planes.size() * 8, hipMemcpyDeviceToHost, c->s));
-            HIPCHK(hipStreamSynchronize(c->s));
+            HIPCHK(hipMemcpyAsync(v.data(), c->val.p, v.size() * 8, hipMemcpyDeviceToHost, c->s));
         }
-        for (size_t b = 0; b < (size_t)c->nnzb; b++)
-            for (int k = 0; k < 9; k++) val[b * 9 + k] = planes[(size_t)k * c->nnzb + b];
     }
     HIPCHK(hipStreamSynchronize(c->s));
+    size_t o = 0;
+    for (int i = 0; i < nv; i++) {
+        if (rowptr) rowptr[i] = (int)o;
+        for (int k = lp[i]; k < lp[i + 1]; k++, o++) {
+            int b = le[2 * k];
+            if (col) col[o] = le[2 * k + 1];
+            if (val)
+                for (int r = 0; r < 3; r++)
+                    for (int cc = 0; cc < 3; cc++) val[o * 9 + 3 * r + cc] = v[(size_t)b * 9 + 3 * cc + r];
+        }
+        for (int b = up[i]; b < up[i + 1]; b++, o++) {
+            if (col) col[o] = ucol[b];
+            if (val)
+                for (int k = 0; k < 9; k++) val[o * 9 + k] = v[(size_t)b * 9 + k];
+        }
+    }
+    if (rowptr) rowptr[nv] = (int)o;
+    return FIPC_OK;
+}
+
+extern "C" int fipc_bsr_storage(fipc_ctx* c, int* nupper, int* nlower)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    if (nupper) *nupper = c->nnzb;
+    if (nlower) *nlower = c->nlower;
     return FIPC_OK;
 }
 

@@ -184,8 +184,14 @@ int fipc_check_collision(fipc_ctx* c, int* hit);
 /* accessors for tests and drivers */
 int fipc_get_vector(fipc_ctx* c, int which, double* out);   /* 0 x, 1 p (solution), 2 rhs, 3 xTilde */
 int fipc_set_vector(fipc_ctx* c, int which, const double* in);
-/* BSR of the elastic + inertia Hessian: rowptr[nv+1], col[nnzb], val[nnzb*9] (block row-major) */
+/* BSR of the elastic + inertia Hessian, both triangles: rowptr[nv+1],
+ * col[nnzb], val[nnzb*9] (block row-major, columns ascending per row).  The
+ * device stores the upper triangle only; this expands it.  With rowptr, col
+ * and val all null only *nnzb is written. */
 int fipc_get_bsr(fipc_ctx* c, int* nnzb, int* rowptr, int* col, double* val);
+/* blocks the device stores (upper triangle incl. diagonal) and the lower
+ * blocks the SpMV reads transposed through (block, column) index pairs */
+int fipc_bsr_storage(fipc_ctx* c, int* nupper, int* nlower);
 /* y = A x with the assembled Hessian (elastic BSR + contact rows), Dirichlet rows zeroed */
 int fipc_spmv(fipc_ctx* c, const double* x, double* y);
 /* timing of the hot kernels with HIP events on the context stream, recorded
