@@ -123,6 +123,9 @@ struct fipc_ctx {
     DVec<int> nuniq;
     // preconditioner + PCG
     DVec<double> cdiag, Minv, r, z, pp, q, qc;   // q = A_bsr p, qc = contact part of A p
+    // FP32 Hessian: the refinement rounds iterate on FP32 copies of the
+    // preconditioner and of r, z, p, q (qc and the solution stay FP64)
+    DVec<float> Minvf, rf, zf, pf, qf;
     DVec<double> part_a, part_b, part_c, scal;
     DVec<int> pcg_state;   // [0] done_at, [1] iterations
     // reductions for energy / residual / ccd

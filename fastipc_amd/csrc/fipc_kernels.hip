@@ -351,7 +351,8 @@ __global__ void __launch_bounds__(TPB) k_contact(int n, const int* __restrict__ 
 // block-Jacobi preconditioner: inverse of (elastic diag block + contact diag
 // block); Dirichlet vertices get the identity (their residual is zero)
 __global__ void k_precond(int nv, const double* __restrict__ val, const int* __restrict__ diag,
-                          const double* __restrict__ cdiag, const unsigned char* __restrict__ dmask, double* Minv)
+                          const double* __restrict__ cdiag, const unsigned char* __restrict__ dmask, double* Minv,
+                          float* Minvf)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nv) return;
@@ -360,14 +361,20 @@ __global__ void k_precond(int nv, const double* __restrict__ val, const int* __r
     for (int k = 0; k < 9; k++) A[k] = val[(size_t)b * 9 + k] + cdiag[9 * i + k];
     double* o = Minv + 9 * i;
     double d = det3(A);
+    double M[9];
     if (dmask[i] || !(fabs(d) > 0.0)) {
-        for (int k = 0; k < 9; k++) o[k] = (k % 4 == 0) ? (dmask[i] ? 1.0 : 1.0 / fmax(fabs(A[k]), 1e-300)) : 0.0;
-        return;
+        for (int k = 0; k < 9; k++) M[k] = (k % 4 == 0) ? (dmask[i] ? 1.0 : 1.0 / fmax(fabs(A[k]), 1e-300)) : 0.0;
+    } else {
+        double id = 1.0 / d;
+        M[0] = (A[4] * A[8] - A[5] * A[7]) * id; M[1] = (A[2] * A[7] - A[1] * A[8]) * id;
+        M[2] = (A[1] * A[5] - A[2] * A[4]) * id; M[3] = (A[5] * A[6] - A[3] * A[8]) * id;
+        M[4] = (A[0] * A[8] - A[2] * A[6]) * id; M[5] = (A[2] * A[3] - A[0] * A[5]) * id;
+        M[6] = (A[3] * A[7] - A[4] * A[6]) * id; M[7] = (A[1] * A[6] - A[0] * A[7]) * id;
+        M[8] = (A[0] * A[4] - A[1] * A[3]) * id;
     }
-    double id = 1.0 / d;
-    o[0] = (A[4] * A[8] - A[5] * A[7]) * id; o[1] = (A[2] * A[7] - A[1] * A[8]) * id; o[2] = (A[1] * A[5] - A[2] * A[4]) * id;
-    o[3] = (A[5] * A[6] - A[3] * A[8]) * id; o[4] = (A[0] * A[8] - A[2] * A[6]) * id; o[5] = (A[2] * A[3] - A[0] * A[5]) * id;
-    o[6] = (A[3] * A[7] - A[4] * A[6]) * id; o[7] = (A[1] * A[6] - A[0] * A[7]) * id; o[8] = (A[0] * A[4] - A[1] * A[3]) * id;
+    for (int k = 0; k < 9; k++) o[k] = M[k];
+    if (Minvf)
+        for (int k = 0; k < 9; k++) Minvf[9 * i + k] = (float)M[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -403,9 +410,9 @@ struct ContactSet {
     int n[NTYPES];
     int boff[NTYPES + 1];
 };
-template <int KIND>
+template <int KIND, class V>
 __device__ __forceinline__ double contact_rows_spmv(int n, const int* __restrict__ rows, const double* __restrict__ Hc,
-                                                    const double* __restrict__ p, double* qc,
+                                                    const V* __restrict__ p, double* qc,
                                                     const unsigned char* __restrict__ dmask, int gid)
 {
     using KI = KindInfo<KIND>;
@@ -429,15 +436,15 @@ __device__ __forceinline__ double contact_rows_spmv(int n, const int* __restrict
 #pragma unroll
         for (int c = 0; c < N; c++) s += Hr[i * N + c] * pv[c];
         atomicAdd(&qc[3 * v + i], s);
-        pq += p[3 * v + i] * s;
+        pq += (double)p[3 * v + i] * s;
     }
     return pq;
 }
-template <class T>
+template <class T, class V>
 __global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ rowptr, const int* __restrict__ col,
                                               const int* __restrict__ lptr, const int2* __restrict__ lent,
-                                              const T* __restrict__ val, const double* __restrict__ p,
-                                              double* __restrict__ q, double* qc,
+                                              const T* __restrict__ val, const V* __restrict__ p,
+                                              V* __restrict__ q, double* qc,
                                               const unsigned char* __restrict__ dmask, ContactSet cs, int nb_c,
                                               int nb_bsr, double* part, const int* __restrict__ state, int it)
 {
@@ -495,7 +502,7 @@ __global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ ro
                     t = v2; v2 = v6; v6 = t;
                     t = v5; v5 = v7; v7 = t;
                 }
-                double x0 = p[3 * c], x1 = p[3 * c + 1], x2 = p[3 * c + 2];
+                double x0 = (double)p[3 * c], x1 = (double)p[3 * c + 1], x2 = (double)p[3 * c + 2];
                 a0 += v0 * x0 + v1 * x1 + v2 * x2;
                 a1 += v3 * x0 + v4 * x1 + v5 * x2;
                 a2 += v6 * x0 + v7 * x1 + v8 * x2;
@@ -508,8 +515,8 @@ __global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ ro
             }
             if (row < nv && lane == 0) {
                 if (dmask[row]) a0 = a1 = a2 = 0.0;
-                q[3 * row] = a0; q[3 * row + 1] = a1; q[3 * row + 2] = a2;
-                pq += p[3 * row] * a0 + p[3 * row + 1] * a1 + p[3 * row + 2] * a2;
+                q[3 * row] = (V)a0; q[3 * row + 1] = (V)a1; q[3 * row + 2] = (V)a2;
+                pq += (double)p[3 * row] * a0 + (double)p[3 * row + 1] * a1 + (double)p[3 * row + 2] * a2;
             }
         }
     } else {
@@ -518,13 +525,13 @@ __global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ ro
         while (t < NTYPES - 1 && cb >= cs.boff[t + 1]) t++;
         int gid = (cb - cs.boff[t]) * TPB + (int)threadIdx.x;
         switch (t) {
-        case 0: pq = contact_rows_spmv<0>(cs.n[0], cs.rows[0], cs.H[0], p, qc, dmask, gid); break;
-        case 1: pq = contact_rows_spmv<1>(cs.n[1], cs.rows[1], cs.H[1], p, qc, dmask, gid); break;
-        case 2: pq = contact_rows_spmv<2>(cs.n[2], cs.rows[2], cs.H[2], p, qc, dmask, gid); break;
-        case 3: pq = contact_rows_spmv<3>(cs.n[3], cs.rows[3], cs.H[3], p, qc, dmask, gid); break;
-        case 4: pq = contact_rows_spmv<4>(cs.n[4], cs.rows[4], cs.H[4], p, qc, dmask, gid); break;
-        case 5: pq = contact_rows_spmv<5>(cs.n[5], cs.rows[5], cs.H[5], p, qc, dmask, gid); break;
-        default: pq = contact_rows_spmv<6>(cs.n[6], cs.rows[6], cs.H[6], p, qc, dmask, gid); break;
+        case 0: pq = contact_rows_spmv<0, V>(cs.n[0], cs.rows[0], cs.H[0], p, qc, dmask, gid); break;
+        case 1: pq = contact_rows_spmv<1, V>(cs.n[1], cs.rows[1], cs.H[1], p, qc, dmask, gid); break;
+        case 2: pq = contact_rows_spmv<2, V>(cs.n[2], cs.rows[2], cs.H[2], p, qc, dmask, gid); break;
+        case 3: pq = contact_rows_spmv<3, V>(cs.n[3], cs.rows[3], cs.H[3], p, qc, dmask, gid); break;
+        case 4: pq = contact_rows_spmv<4, V>(cs.n[4], cs.rows[4], cs.H[4], p, qc, dmask, gid); break;
+        case 5: pq = contact_rows_spmv<5, V>(cs.n[5], cs.rows[5], cs.H[5], p, qc, dmask, gid); break;
+        default: pq = contact_rows_spmv<6, V>(cs.n[6], cs.rows[6], cs.H[6], p, qc, dmask, gid); break;
         }
     }
     if (part) {
@@ -548,10 +555,12 @@ __global__ void k_pcg_x0(int nv, const double* xt, const double* xcur, const uns
         qc[3 * i + d] = 0.0;
     }
 }
-// r = b - (q + qc) (= b - A x); z = Minv r; p = z; qc cleared for the
-// iterations; partials of r.z and r.r
+// r = b - (q + qc) (= b - A x, from FP64 products); z = Minv r; p = z; qc
+// cleared for the iterations; partials of r.z and r.r.  V is the type the
+// iterations keep r, z, p in (float for the FP32 Hessian's refinement rounds).
+template <class V>
 __global__ void k_pcg_init(int nv, const double* b, const double* q, double* qc, const unsigned char* dmask,
-                           const double* Minv, double* r, double* z, double* p, double* part_rz, double* part_bb)
+                           const double* Minv, V* r, V* z, V* p, double* part_rz, double* part_bb)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0, bb = 0.0;
@@ -562,8 +571,9 @@ __global__ void k_pcg_init(int nv, const double* b, const double* q, double* qc,
         const double* M = Minv + 9 * i;
         for (int d = 0; d < 3; d++) zi[d] = fixed ? 0.0 : M[3 * d] * ri[0] + M[3 * d + 1] * ri[1] + M[3 * d + 2] * ri[2];
         for (int d = 0; d < 3; d++) {
-            r[3 * i + d] = ri[d]; z[3 * i + d] = zi[d]; p[3 * i + d] = zi[d]; qc[3 * i + d] = 0.0;
-            rz += ri[d] * zi[d];
+            V rv = (V)ri[d], zv = (V)zi[d];
+            r[3 * i + d] = rv; z[3 * i + d] = zv; p[3 * i + d] = zv; qc[3 * i + d] = 0.0;
+            rz += (double)rv * (double)zv;
             bb += ri[d] * ri[d];
         }
     }
@@ -600,11 +610,14 @@ __global__ void k_pcg_scalars(const double* part_rz, const double* part_bb, int 
 #define PCG_MAXB 512
 static inline int pcg_grid(int nv) { return std::min(nblk(nv), PCG_MAXB); }
 // alpha = rz / pq (pq from the SpMV partials); x += alpha p; r -= alpha q;
-// z = Minv r; partials of r.z and r.r
+// z = Minv r; partials of r.z and r.r.  V / M: storage types of the
+// iteration vectors and of the preconditioner blocks (the arithmetic is FP64;
+// x is always FP64)
+template <class V, class M>
 __global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const unsigned char* __restrict__ dmask,
-                                                  const double* __restrict__ Minv, const double* __restrict__ p,
-                                                  const double* __restrict__ q, const double* __restrict__ qc,
-                                                  double* x, double* r, double* z,
+                                                  const M* __restrict__ Minv, const V* __restrict__ p,
+                                                  const V* __restrict__ q, const double* __restrict__ qc,
+                                                  double* x, V* r, V* z,
                                                   const double* part_pq, int npq, double* part_rz, double* part_rr,
                                                   const int* state, int it)
 {
@@ -617,15 +630,16 @@ __global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const un
         if (dmask[i]) continue;
         double ri[3];
         for (int d = 0; d < 3; d++) {
-            x[3 * i + d] += alpha * p[3 * i + d];
-            ri[d] = r[3 * i + d] - alpha * (q[3 * i + d] + qc[3 * i + d]);
-            r[3 * i + d] = ri[d];
+            x[3 * i + d] += alpha * (double)p[3 * i + d];
+            V rv = (V)((double)r[3 * i + d] - alpha * ((double)q[3 * i + d] + qc[3 * i + d]));
+            r[3 * i + d] = rv;
+            ri[d] = (double)rv;
         }
-        const double* M = Minv + 9 * i;
+        const M* Mi = Minv + 9 * i;
         for (int d = 0; d < 3; d++) {
-            double zd = M[3 * d] * ri[0] + M[3 * d + 1] * ri[1] + M[3 * d + 2] * ri[2];
-            z[3 * i + d] = zd;
-            rz += ri[d] * zd;
+            V zv = (V)((double)Mi[3 * d] * ri[0] + (double)Mi[3 * d + 1] * ri[1] + (double)Mi[3 * d + 2] * ri[2]);
+            z[3 * i + d] = zv;
+            rz += ri[d] * (double)zv;
             rr += ri[d] * ri[d];
         }
     }
@@ -638,7 +652,8 @@ __global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const un
 }
 // reduces r.z and r.r, runs the convergence test (block 0 records it),
 // beta = rz_new / rz; p = z + beta p; qc = 0 for the next product
-__global__ void __launch_bounds__(TPB) k_pcg_upd2(int nv, double* scal, const double* __restrict__ z, double* p,
+template <class V>
+__global__ void __launch_bounds__(TPB) k_pcg_upd2(int nv, double* scal, const V* __restrict__ z, V* p,
                                                   double* qc, const double* part_rz, const double* part_rr, int n1,
                                                   int* state, int it, double tol2)
 {
@@ -653,7 +668,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_upd2(int nv, double* scal, const do
         if (rr <= tol2 * scal[2] || !(rr == rr)) state[0] = it;
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nv; i += gridDim.x * blockDim.x) {
-        p[i] = z[i] + beta * p[i];
+        p[i] = (V)((double)z[i] + beta * (double)p[i]);
         qc[i] = 0.0;
     }
 }
@@ -835,6 +850,7 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     for (int t = 0; t < NTYPES; t++) { release(c->rows[t]); release(c->cH[t]); }
     release(c->keys_pp); release(c->keys_pe); release(c->keys_tmp); release(c->counters); release(c->cub_tmp);
     release(c->nuniq); release(c->cdiag); release(c->Minv); release(c->r); release(c->z); release(c->pp); release(c->q); release(c->qc);
+    release(c->Minvf); release(c->rf); release(c->zf); release(c->pf); release(c->qf);
     release(c->part_a); release(c->part_b); release(c->part_c); release(c->scal); release(c->pcg_state);
     release(c->epart); release(c->amin); release(c->flag);
     release(c->grid.cnt); release(c->grid.ekey); release(c->grid.eitem); release(c->grid.bstart);
@@ -1140,7 +1156,9 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     STCHK(fipc_tm_begin(c, TM_CONTACT));
     STCHK(launch_contacts(c, 1));
     STCHK(fipc_tm_end(c, TM_CONTACT));
-    k_precond<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->val.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p);
+    if (precision == 1) STCHK(ensure(c->Minvf, (size_t)c->nv * 9));
+    k_precond<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->val.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p,
+                                             precision == 1 ? c->Minvf.p : nullptr);
     HIPCHK(hipGetLastError());
     return FIPC_OK;
 }
@@ -1162,8 +1180,9 @@ static int spmv_contacts(fipc_ctx* c, ContactSet& cs)
     return (nb + 7) & ~7;
 }
 // q + qc = A p (q = BSR part, qc = contact part accumulated: qc must be zero
-// on entry); prec selects the FP32 copy of the BSR values
-static int spmv(fipc_ctx* c, int prec, const double* p, double* q, double* qc, double* part, const int* state, int it)
+// on entry); prec selects the FP32 copy of the BSR values, V the type of p, q
+template <class V>
+static int spmv(fipc_ctx* c, int prec, const V* p, V* q, double* qc, double* part, const int* state, int it)
 {
     ContactSet cs;
     int nbc = spmv_contacts(c, cs);
@@ -1173,11 +1192,11 @@ static int spmv(fipc_ctx* c, int prec, const double* p, double* q, double* qc, d
     const int timer = part ? TM_SPMV : TM_SPMV_RES;
     STCHK(fipc_tm_begin(c, timer));
     if (prec == 1)
-        k_spmv<float><<<nbc + nbb, TPB, 0, c->s>>>(c->nv, c->rowptr.p, c->col.p, c->lptr.p, (const int2*)c->lent.p,
-                                                   c->valf.p, p, q, qc, c->dmask.p, cs, nbc, nbb, part, state, it);
+        k_spmv<float, V><<<nbc + nbb, TPB, 0, c->s>>>(c->nv, c->rowptr.p, c->col.p, c->lptr.p, (const int2*)c->lent.p,
+                                                      c->valf.p, p, q, qc, c->dmask.p, cs, nbc, nbb, part, state, it);
     else
-        k_spmv<double><<<nbc + nbb, TPB, 0, c->s>>>(c->nv, c->rowptr.p, c->col.p, c->lptr.p, (const int2*)c->lent.p,
-                                                    c->val.p, p, q, qc, c->dmask.p, cs, nbc, nbb, part, state, it);
+        k_spmv<double, V><<<nbc + nbb, TPB, 0, c->s>>>(c->nv, c->rowptr.p, c->col.p, c->lptr.p, (const int2*)c->lent.p,
+                                                       c->val.p, p, q, qc, c->dmask.p, cs, nbc, nbb, part, state, it);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, timer));
     return FIPC_OK;
@@ -1207,21 +1226,75 @@ extern "C" int fipc_spmv(fipc_ctx* c, const double* x, double* y)
     return FIPC_OK;
 }
 
+// the iterations of one PCG round on vectors of type V (FP64 with the FP64
+// Hessian; FP32 in the FP32 Hessian's refinement rounds)
+template <class V, class M>
+static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* q, int round, int budget,
+                     double tol2, int state[2])
+{
+    const int nv = c->nv, nbv = nblk(nv), nsp = spmv_nparts(c), nu = pcg_grid(nv), chunk = 8;
+    // r = b - A x with the FP64 blocks (q + qc), z = Minv r, p = z
+    STCHK(spmv<double>(c, 0, c->sol.p, c->q.p, c->qc.p, nullptr, nullptr, 0));
+    k_pcg_init<V><<<nbv, TPB, 0, c->s>>>(nv, c->rhs.p, c->q.p, c->qc.p, c->dmask.p, c->Minv.p, r, z, p, c->part_a.p,
+                                         c->part_b.p);
+    HIPCHK(hipGetLastError());
+    k_pcg_scalars<<<1, TPB, 0, c->s>>>(c->part_a.p, c->part_b.p, nbv, c->scal.p, c->pcg_state.p, round > 0, tol2);
+    HIPCHK(hipGetLastError());
+    int it = 0;
+    state[0] = 0x7fffffff;
+    state[1] = 0;
+    if (round > 0) {
+        // the recomputed residual may already meet the tolerance
+        HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        if (state[0] == -1) return FIPC_OK;
+    }
+    // the host checks convergence between chunks of iterations; the first
+    // chunk is sized from the previous solve, whose count is usually close
+    int first = round == 0 ? std::max(chunk, c->pcg_hint - chunk / 2) : chunk;
+    while (it < budget) {
+        int end = std::min(budget, it + (it == 0 ? first : chunk));
+        for (; it < end; it++) {
+            STCHK(spmv<V>(c, prec, p, q, c->qc.p, c->part_c.p, c->pcg_state.p, it));
+            STCHK(fipc_tm_begin(c, TM_PCGVEC));
+            k_pcg_upd1<V, M><<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->dmask.p, Minv, p, q, c->qc.p, c->sol.p, r, z,
+                                                   c->part_c.p, nsp, c->part_a.p, c->part_b.p, c->pcg_state.p, it);
+            k_pcg_upd2<V><<<nu, TPB, 0, c->s>>>(nv, c->scal.p, z, p, c->qc.p, c->part_a.p, c->part_b.p, nu,
+                                                c->pcg_state.p, it, tol2);
+            STCHK(fipc_tm_end(c, TM_PCGVEC));
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        if (state[0] != 0x7fffffff) break;
+    }
+    if (budget <= 0) {
+        HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+    }
+    return FIPC_OK;
+}
+
 // solve_system (NEWTON_IPC.py:750-777): block-Jacobi PCG from the prescribed
 // Dirichlet increments.  With the FP32 Hessian the iterations use the FP32
-// copy of the elastic blocks and are restarted from the residual b - A x
-// recomputed with the FP64 blocks until that residual meets the tolerance
-// (iterative refinement); with the FP64 Hessian there is a single round.
+// copy of the elastic blocks, of the preconditioner and of the iteration
+// vectors, and are restarted from the residual b - A x recomputed with the
+// FP64 blocks until that residual meets the tolerance (iterative
+// refinement; the solution accumulates in FP64); with the FP64 Hessian there
+// is a single FP64 round.
 extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double* relres)
 {
     if (!c || !c->nv) return FIPC_ERR_STATE;
     int nv = c->nv;
     int nbv = nblk(nv);
-    const int chunk = 8;
-    const int nsp = spmv_nparts(c), nu = pcg_grid(nv);
+    const int nsp = spmv_nparts(c);
     const double tol2 = tol * tol;
     const int prec = c->precision;
     STCHK(ensure(c->part_c, (size_t)nsp + 64));
+    if (prec == 1) {
+        STCHK(ensure(c->rf, (size_t)nv * 3)); STCHK(ensure(c->zf, (size_t)nv * 3));
+        STCHK(ensure(c->pf, (size_t)nv * 3)); STCHK(ensure(c->qf, (size_t)nv * 3));
+    }
     const int rounds = prec == 1 ? 4 : 1;
     int total = 0, state[2] = {0, 0};
     bool converged = false;
@@ -1233,45 +1306,14 @@ extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double
         } else {
             HIPCHK(hipMemsetAsync(c->qc.p, 0, (size_t)nv * 24, c->s));
         }
-        // q + qc = A x with the FP64 blocks; r = b - A x, z = Minv r, p = z
-        STCHK(spmv(c, 0, c->sol.p, c->q.p, c->qc.p, nullptr, nullptr, 0));
-        k_pcg_init<<<nbv, TPB, 0, c->s>>>(nv, c->rhs.p, c->q.p, c->qc.p, c->dmask.p, c->Minv.p, c->r.p, c->z.p, c->pp.p,
-                                           c->part_a.p, c->part_b.p);
-        HIPCHK(hipGetLastError());
-        k_pcg_scalars<<<1, TPB, 0, c->s>>>(c->part_a.p, c->part_b.p, nbv, c->scal.p, c->pcg_state.p, round > 0, tol2);
-        HIPCHK(hipGetLastError());
-        int it = 0, budget = maxit - total;
-        state[0] = 0x7fffffff;
-        if (round > 0) {
-            // the recomputed residual may already meet the tolerance
-            HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
-            HIPCHK(hipStreamSynchronize(c->s));
-            if (state[0] == -1) { converged = true; break; }
-        }
-        // the host checks convergence between chunks of iterations; the first
-        // chunk is sized from the previous solve, whose count is usually close
-        int first = round == 0 ? std::max(chunk, c->pcg_hint - chunk / 2) : chunk;
-        while (it < budget) {
-            int end = std::min(budget, it + (it == 0 ? first : chunk));
-            for (; it < end; it++) {
-                STCHK(spmv(c, prec, c->pp.p, c->q.p, c->qc.p, c->part_c.p, c->pcg_state.p, it));
-                STCHK(fipc_tm_begin(c, TM_PCGVEC));
-                k_pcg_upd1<<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->dmask.p, c->Minv.p, c->pp.p, c->q.p, c->qc.p, c->sol.p,
-                                                  c->r.p, c->z.p, c->part_c.p, nsp, c->part_a.p, c->part_b.p,
-                                                  c->pcg_state.p, it);
-                k_pcg_upd2<<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->z.p, c->pp.p, c->qc.p, c->part_a.p, c->part_b.p, nu,
-                                                  c->pcg_state.p, it, tol2);
-                STCHK(fipc_tm_end(c, TM_PCGVEC));
-            }
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
-            HIPCHK(hipStreamSynchronize(c->s));
-            if (state[0] != 0x7fffffff) break;
-        }
-        if (budget <= 0) {
-            HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
-            HIPCHK(hipStreamSynchronize(c->s));
-        }
+        int budget = maxit - total;
+        if (prec == 1)
+            STCHK((pcg_round<float, float>(c, prec, c->Minvf.p, c->rf.p, c->zf.p, c->pf.p, c->qf.p, round, budget,
+                                           tol2, state)));
+        else
+            STCHK((pcg_round<double, double>(c, prec, c->Minv.p, c->r.p, c->z.p, c->pp.p, c->q.p, round, budget,
+                                             tol2, state)));
+        if (state[0] == -1 && round > 0) { converged = true; break; }
         int ran = state[0] == -1 ? 0 : state[1];
         total += ran;
         if (round == 0 && ran > 0) c->pcg_hint = ran;
