@@ -165,7 +165,7 @@ __host__ __device__ constexpr int uslot(int a, int c) { return a * 4 - a * (a - 
 // writes its ten upper 3x3 blocks to tetH[(slot * nt + e) * 9 + k] and its
 // gradient to tetg[(corner * nt + e) * 3 + i]; slot-major layout so that the
 // lanes of a wave store contiguous memory
-__global__ void __launch_bounds__(TPB) k_elastic(int nt, const int* __restrict__ tets, const double* __restrict__ IB,
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) k_elastic(int nt, const int* __restrict__ tets, const double* __restrict__ IB,
                                                  const double* __restrict__ vol, const double* __restrict__ la,
                                                  const double* __restrict__ mu, const double* __restrict__ x,
                                                  double dt2, const int* __restrict__ cpos, double* __restrict__ tetH,
@@ -253,99 +253,257 @@ __global__ void k_gather_rhs(int nv, const int* __restrict__ vcptr, const int* _
     rhs[3 * i] = r0; rhs[3 * i + 1] = r1; rhs[3 * i + 2] = r2;
 }
 
-// contact rows: a group of G lanes per row (G = 2 for PP, 4 otherwise).
-// Every lane evaluates the constraint and runs the Jacobi sweeps on its own
-// copy of the packed Hessian; lane j owns eigenvector rows 3j..3j+2 and
-// writes rows 3j..3j+2 of the projected Hessian (makePD semantics, UTILS.h:7-25).
+// new position of position k after a round of the round-robin ("circle")
+// schedule on M (even) positions, which always pairs positions (i, M-1-i):
+// position 0 stays, the others rotate by one, so the M - 1 rounds of a sweep
+// pair every two indices exactly once
+template <int M>
+__host__ __device__ constexpr int rr_next(int k)
+{
+    return k == 0 ? 0 : (k == M - 1 ? 1 : k + 1);
+}
+// orders LDS accesses between the lanes of one wavefront
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// contact rows: a group of G lanes per row (G = 8 for PP, 16 otherwise).
+// The projection onto the positive semi-definite cone (makePD semantics,
+// UTILS.h:7-25: V max(L, 0) V^T of the eigen-decomposition) runs a parallel
+// cyclic Jacobi on the matrix padded to M = N rounded up to even (a zero row
+// and column, never rotated): lane l < M owns row l, a round applies the M / 2
+// disjoint rotations of the pairs (i, M-1-i) at once, A <- J^T A J and
+// V <- V J, then relabels the indices by the round-robin rotation rr_next, so
+// every round is the same code with static register indices (the Brent-Luk
+// arrangement); rows move between lanes, and the columns of J are exchanged,
+// through LDS.  A pair rotates when |a_pq| > 1e-17 |A|_F; sweeps (M - 1
+// rounds) stop once a whole sweep rotates no pair, after at most max_sweeps.
+// LDS of one contact-assembly block: per lane group the M x M matrix and the
+// two coefficient columns, sized for the largest row (M = 12, G = 16)
+#define CONTACT_LDS ((TPB / 16) * (144 + 2 * 12))
 template <int KIND>
-__global__ void __launch_bounds__(TPB) k_contact(int n, const int* __restrict__ rows, const double* __restrict__ x,
-                                                 const double* __restrict__ x0, double dHat2, double kappa,
-                                                 double* __restrict__ Hout, double* rhs, double* cdiag,
-                                                 double* Eout, double* gout, int project, int accumulate,
-                                                 const unsigned char* __restrict__ dmask)
+__device__ __forceinline__ void contact_row(int n, const int* __restrict__ rows, const double* __restrict__ x,
+                                            const double* __restrict__ x0, double dHat2, double kappa,
+                                            double* __restrict__ Hout, double* rhs, double* cdiag, double* Eout,
+                                            double* gout, int project, int accumulate,
+                                            const unsigned char* __restrict__ dmask, int max_sweeps, int gid,
+                                            double* lds)
 {
     using KI = KindInfo<KIND>;
-    constexpr int N = KI::N, G = KI::G, K = KI::K;
-    int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    int r = gid / G, lane = gid % G;
-    if (r >= n) return;
+    constexpr int N = KI::N, G = KI::G;
+    constexpr int M = (N + 1) & ~1;
+    int r = gid / G, l = gid % G, grp = (int)threadIdx.x / G;
+    if (r >= n) return;  // uniform over the lanes of a row
+    const bool act = l < N;
+    double* A = lds + grp * (M * M + 2 * M);
+    double* jd = A + M * M;
+    double* jo = jd + M;
     int id[4];
     v3 px[4], p0[4];
     load_row<KIND>(rows, r, x, x0, id, px, p0);
     double g[N], H[N * N];
     double E = constraint_gH<KIND>(px, p0, dHat2, kappa, g, H);
-    double A[N * (N + 1) / 2];
+    // own row, from the upper triangle
+    double o[N];
 #pragma unroll
-    for (int i = 0; i < N; i++)
+    for (int k = 0; k < N; k++) o[k] = 0.0;
 #pragma unroll
-        for (int j = i; j < N; j++) A[pk(N, i, j)] = H[i * N + j];
-    double V[3][N];
-    double out[3][N];
+    for (int L = 0; L < N; L++)
+        if (L == l) {
+#pragma unroll
+            for (int k = 0; k < N; k++) o[k] = k >= L ? H[L * N + k] : H[k * N + L];
+        }
+    double out[N];
     if (project) {
-        jacobi_eig<N, 3>(A, V, lane * 3, 20);
-        double lam[N];
+        // derivative-only arithmetic: contraction into FMA is allowed here
+#pragma clang fp contract(fast)
+        const bool pos = l < M;
+        double a[M], v[M];
 #pragma unroll
-        for (int k = 0; k < N; k++) lam[k] = fmax(A[pk(N, k, k)], 0.0);
-        double lv[3][N];
+        for (int k = 0; k < M; k++) {
+            a[k] = k < N ? o[k] : 0.0;
+            v[k] = k == l ? 1.0 : 0.0;
+        }
+        double f = 0.0;
 #pragma unroll
-        for (int i = 0; i < 3; i++)
+        for (int k = 0; k < M; k++) f += a[k] * a[k];
 #pragma unroll
-            for (int k = 0; k < N; k++) { lv[i][k] = lam[k] * V[i][k]; out[i][k] = 0.0; }
-        // rows c of V live in lane c / 3 of this group
-        int base = (threadIdx.x / G) * G;
+        for (int mm = G / 2; mm > 0; mm >>= 1) f += __shfl_xor(f, mm, G);
+        const double tiny = 1e-17 * sqrt(f);
+        if (pos) {
 #pragma unroll
-        for (int c = 0; c < N; c++) {
-            double vc[N];
+            for (int k = 0; k < M; k++) A[l * M + k] = a[k];
+        }
+        wave_sync();
+        const unsigned long long gmask = ((1ull << G) - 1) << (((int)threadIdx.x % 64) / G * G);
+        const int m = M - 1 - l;
+        const int dest = l == 0 ? 0 : (l == M - 1 ? 1 : l + 1);
+        for (int sweep = 0; sweep < max_sweeps; sweep++) {
+            bool rot = false;
+#pragma unroll 1
+            for (int rd = 0; rd < M - 1; rd++) {
+                double c = 1.0, s = 0.0, t = 0.0, app = 0.0, aqq = 0.0, apq = 0.0;
+                double w[M];
 #pragma unroll
-            for (int k = 0; k < N; k++) vc[k] = __shfl(V[c % 3][k], base + c / 3, 64);
+                for (int k = 0; k < M; k++) w[k] = 0.0;
+                if (pos) {
+                    int p = min(l, m), q = max(l, m);
+                    app = A[p * M + p];
+                    aqq = A[q * M + q];
+                    apq = A[p * M + q];
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                double s = 0.0;
+                    for (int k = 0; k < M; k++) w[k] = A[m * M + k];
+                }
+                const bool did = pos && fabs(apq) > tiny;
+                if (did) {
+                    double theta = (aqq - app) / (2.0 * apq);
+                    t = fabs(theta) > 1e150 ? 0.5 / theta
+                      : (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    c = 1.0 / sqrt(t * t + 1.0);
+                    s = t * c;
+                }
+                // column l of J: J[l][l] = c, J[m][l] = -s (l < m) or s (l > m)
+                const double cl = c, sl = l < m ? -s : s;
+                if (pos) { jd[l] = cl; jo[l] = sl; }
+                wave_sync();
+                double u[M];
 #pragma unroll
-                for (int k = 0; k < N; k++) s += lv[i][k] * vc[k];
-                out[i][c] = s;
+                for (int k = 0; k < M; k++) u[k] = cl * a[k] + sl * w[k];
+                double an[M], vn[M];
+#pragma unroll
+                for (int k = 0; k < M; k++) {
+                    double ck = jd[k], sk = jo[k];
+                    an[k] = ck * u[k] + sk * u[M - 1 - k];
+                    vn[k] = ck * v[k] + sk * v[M - 1 - k];
+                }
+                if (did) {
+                    // the rotated pair's entries, exactly
+                    double dnew = l < m ? app - t * apq : aqq + t * apq;
+#pragma unroll
+                    for (int k = 0; k < M; k++) {
+                        if (k == m) an[k] = 0.0;
+                        if (k == l) an[k] = dnew;
+                    }
+                }
+                rot = rot || did;
+                // relabel the columns (static) and move the row to its new position
+#pragma unroll
+                for (int k = 0; k < M; k++) {
+                    a[rr_next<M>(k)] = an[k];
+                    v[rr_next<M>(k)] = vn[k];
+                }
+                if (pos) {
+#pragma unroll
+                    for (int k = 0; k < M; k++) A[dest * M + k] = a[k];
+                }
+                wave_sync();
+                if (pos) {
+#pragma unroll
+                    for (int k = 0; k < M; k++) a[k] = A[l * M + k];
+                }
             }
+            if ((__ballot(rot) & gmask) == 0) break;
+        }
+        // eigenvalues clamped at zero; rows of V replace A in LDS
+        double lam = pos ? fmax(A[l * M + l], 0.0) : 0.0;
+        wave_sync();
+        if (pos) {
+#pragma unroll
+            for (int k = 0; k < M; k++) A[l * M + k] = v[k];
+            jd[l] = lam;
+        }
+        wave_sync();
+        double lv[M];
+#pragma unroll
+        for (int k = 0; k < M; k++) lv[k] = v[k] * jd[k];
+#pragma unroll
+        for (int cc = 0; cc < N; cc++) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < M; k++) sum += lv[k] * A[cc * M + k];
+            out[cc] = sum;
         }
     } else {
-        // rows of the unprojected Hessian owned by this lane (static indices)
 #pragma unroll
-        for (int L = 0; L < K; L++)
-            if (L == lane) {
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-#pragma unroll
-                    for (int c = 0; c < N; c++) out[i][c] = H[(L * 3 + i) * N + c];
-            }
+        for (int k = 0; k < N; k++) out[k] = o[k];
     }
-    if (lane >= K) return;
-    double* Hr = Hout + (size_t)r * N * N + lane * 3 * N;
+    if (!act) return;
+    double* Hr = Hout + (size_t)r * N * N + (size_t)l * N;
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+    for (int k = 0; k < N; k++) Hr[k] = out[k];
+    const int L = l / 3, i = l % 3;
+    int vid = id[0];
+    double gl = g[0];
 #pragma unroll
-        for (int c = 0; c < N; c++) Hr[i * N + c] = out[i][c];
+    for (int a = 1; a < 4; a++)
+        if (a == L) vid = id[a];
 #pragma unroll
-    for (int L = 0; L < K; L++) {
-        if (L != lane) continue;
-        if (accumulate) {
-            int v = id[L];
-            // Dirichlet rows are replaced by the identity in the solve: skip
-            // them (a fixed plate touches thousands of rows at once)
-            if (dmask[v]) continue;
-            // load_hessian_and_gradient (NEWTON_IPC.py:545-555): rhs -= g
+    for (int k = 1; k < N; k++)
+        if (k == l) gl = g[k];
+    if (accumulate) {
+        // Dirichlet rows are replaced by the identity in the solve: skip them
+        // (a fixed plate touches thousands of rows at once)
+        if (dmask[vid]) return;
+        // load_hessian_and_gradient (NEWTON_IPC.py:545-555): rhs -= g, and the
+        // diagonal 3x3 block of the vertex for the preconditioner
+        atomicAdd(&rhs[3 * vid + i], -gl);
 #pragma unroll
-            for (int i = 0; i < 3; i++) atomicAdd(&rhs[3 * v + i], -g[L * 3 + i]);
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) atomicAdd(&cdiag[9 * v + i * 3 + j], out[i][L * 3 + j]);
-        } else {
-            if (L == 0 && Eout) Eout[r] = E;
-            if (gout) {
-#pragma unroll
-                for (int i = 0; i < 3; i++) gout[(size_t)r * N + L * 3 + i] = g[L * 3 + i];
-            }
-        }
+        for (int k = 0; k < N; k++)
+            if (k / 3 == L) atomicAdd(&cdiag[9 * vid + 3 * i + k % 3], out[k]);
+    } else {
+        if (l == 0 && Eout) Eout[r] = E;
+        if (gout) gout[(size_t)r * N + l] = gl;
     }
+}
+
+// every constraint type in one launch: blocks [boff[t], boff[t + 1]) run the
+// rows of type t, so the (latency-bound) types overlap instead of queueing
+struct ContactBatch {
+    const int* rows[NTYPES];
+    double* H[NTYPES];
+    int n[NTYPES];
+    int boff[NTYPES + 1];
+};
+__global__ void __launch_bounds__(TPB) k_contact(ContactBatch cb, const double* __restrict__ x,
+                                                 const double* __restrict__ x0, double dHat2, double kappa, double* rhs,
+                                                 double* cdiag, double* Eout, double* gout, int project,
+                                                 int accumulate, const unsigned char* __restrict__ dmask,
+                                                 int max_sweeps)
+{
+    __shared__ double lds[CONTACT_LDS];
+    int b = (int)blockIdx.x, t = 0;
+    while (t < NTYPES - 1 && b >= cb.boff[t + 1]) t++;
+    int gid = (b - cb.boff[t]) * TPB + (int)threadIdx.x;
+#define CONTACT_CASE(K)                                                                                            \
+    case K:                                                                                                         \
+        contact_row<K>(cb.n[K], cb.rows[K], x, x0, dHat2, kappa, cb.H[K], rhs, cdiag, Eout, gout, project,         \
+                       accumulate, dmask, max_sweeps, gid, lds);                                                    \
+        break;
+    switch (t) {
+        CONTACT_CASE(0)
+        CONTACT_CASE(1)
+        CONTACT_CASE(2)
+        CONTACT_CASE(3)
+        CONTACT_CASE(4)
+        CONTACT_CASE(5)
+        CONTACT_CASE(6)
+    }
+#undef CONTACT_CASE
+}
+// block ranges of the types with rows (G lanes per row)
+static int contact_blocks(ContactBatch& cb)
+{
+    int nb = 0;
+    for (int t = 0; t < NTYPES; t++) {
+        int G = kind_dofs(t) <= 8 ? 8 : 16;
+        cb.boff[t] = nb;
+        nb += cb.n[t] ? nblk((long)cb.n[t] * G) : 0;
+    }
+    cb.boff[NTYPES] = nb;
+    return nb;
 }
 
 // block-Jacobi preconditioner: inverse of (elastic diag block + contact diag
@@ -616,7 +774,7 @@ static inline int pcg_grid(int nv) { return std::min(nblk(nv), PCG_MAXB); }
 template <class V, class M>
 __global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const unsigned char* __restrict__ dmask,
                                                   const M* __restrict__ Minv, const V* __restrict__ p,
-                                                  const V* __restrict__ q, const double* __restrict__ qc,
+                                                  const V* __restrict__ q, double* __restrict__ qc,
                                                   double* x, V* r, V* z,
                                                   const double* part_pq, int npq, double* part_rz, double* part_rr,
                                                   const int* state, int it)
@@ -631,7 +789,11 @@ __global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const un
         double ri[3];
         for (int d = 0; d < 3; d++) {
             x[3 * i + d] += alpha * (double)p[3 * i + d];
-            V rv = (V)((double)r[3 * i + d] - alpha * ((double)q[3 * i + d] + qc[3 * i + d]));
+            // qc is non-zero only at contact vertices: cleared here, where
+            // it is consumed, for the next product
+            double qcv = qc[3 * i + d];
+            if (qcv != 0.0) qc[3 * i + d] = 0.0;
+            V rv = (V)((double)r[3 * i + d] - alpha * ((double)q[3 * i + d] + qcv));
             r[3 * i + d] = rv;
             ri[d] = (double)rv;
         }
@@ -651,10 +813,10 @@ __global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const un
     }
 }
 // reduces r.z and r.r, runs the convergence test (block 0 records it),
-// beta = rz_new / rz; p = z + beta p; qc = 0 for the next product
+// beta = rz_new / rz; p = z + beta p
 template <class V>
 __global__ void __launch_bounds__(TPB) k_pcg_upd2(int nv, double* scal, const V* __restrict__ z, V* p,
-                                                  double* qc, const double* part_rz, const double* part_rr, int n1,
+                                                  const double* part_rz, const double* part_rr, int n1,
                                                   int* state, int it, double tol2)
 {
     if (it > state[0]) return;
@@ -667,10 +829,8 @@ __global__ void __launch_bounds__(TPB) k_pcg_upd2(int nv, double* scal, const V*
         state[1] = it + 1;
         if (rr <= tol2 * scal[2] || !(rr == rr)) state[0] = it;
     }
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nv; i += gridDim.x * blockDim.x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nv; i += gridDim.x * blockDim.x)
         p[i] = (V)((double)z[i] + beta * (double)p[i]);
-        qc[i] = 0.0;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -728,14 +888,21 @@ __global__ void k_min_dist(int n, const int* rows, const double* x, unsigned lon
     atomicMin(out, (unsigned long long)__double_as_longlong(d));
 }
 
-// inversion filter (NEWTON_IPC.py:443-453)
+// inversion filter (NEWTON_IPC.py:443-453): the minimum over the tets, folded
+// into *amin (a non-negative double's bit pattern) with one atomic per
+// wavefront
 __global__ void k_inversion(int nt, const int* tets, const double* x, const double* p, unsigned long long* amin)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nt) return;
-    int a = tets[4 * e], b = tets[4 * e + 1], c = tets[4 * e + 2], d = tets[4 * e + 3];
-    double al = tet_inversion_alpha(ldv(x, a), ldv(x, b), ldv(x, c), ldv(x, d), ldv(p, a), ldv(p, b), ldv(p, c), ldv(p, d), 0.2);
-    atomicMin(amin, (unsigned long long)__double_as_longlong(al));
+    double al = 1.0;
+    if (e < nt) {
+        int a = tets[4 * e], b = tets[4 * e + 1], c = tets[4 * e + 2], d = tets[4 * e + 3];
+        al = tet_inversion_alpha(ldv(x, a), ldv(x, b), ldv(x, c), ldv(x, d), ldv(p, a), ldv(p, b), ldv(p, c),
+                                 ldv(p, d), 0.2);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) al = fmin(al, __shfl_xor(al, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMin(amin, (unsigned long long)__double_as_longlong(al));
 }
 
 // batched tet evaluation for tests
@@ -1113,25 +1280,17 @@ extern "C" int fipc_residual(fipc_ctx* c, double* res)
 // ---- assembly ----
 static int launch_contacts(fipc_ctx* c, int accumulate)
 {
+    ContactBatch cb;
     for (int t = 0; t < NTYPES; t++) {
-        int n = c->count[t];
-        if (!n) continue;
-        int G = t == 0 ? 2 : 4;
-        int nb = nblk((long)n * G);
-        double* H = c->cH[t].p;
-#define LAUNCH_C(K) k_contact<K><<<nb, TPB, 0, c->s>>>(n, c->rows[t].p, c->x.p, c->x0.p, c->dHat2, c->kappa, H, c->rhs.p, c->cdiag.p, nullptr, nullptr, 1, accumulate, c->dmask.p)
-        switch (t) {
-        case 0: LAUNCH_C(0); break;
-        case 1: LAUNCH_C(1); break;
-        case 2: LAUNCH_C(2); break;
-        case 3: LAUNCH_C(3); break;
-        case 4: LAUNCH_C(4); break;
-        case 5: LAUNCH_C(5); break;
-        default: LAUNCH_C(6); break;
-        }
-#undef LAUNCH_C
-        HIPCHK(hipGetLastError());
+        cb.rows[t] = c->rows[t].p;
+        cb.H[t] = c->cH[t].p;
+        cb.n[t] = c->count[t];
     }
+    int nb = contact_blocks(cb);
+    if (!nb) return FIPC_OK;
+    k_contact<<<nb, TPB, 0, c->s>>>(cb, c->x.p, c->x0.p, c->dHat2, c->kappa, c->rhs.p, c->cdiag.p, nullptr, nullptr, 1,
+                                    accumulate, c->dmask.p, 20);
+    HIPCHK(hipGetLastError());
     return FIPC_OK;
 }
 
@@ -1259,8 +1418,8 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
             STCHK(fipc_tm_begin(c, TM_PCGVEC));
             k_pcg_upd1<V, M><<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->dmask.p, Minv, p, q, c->qc.p, c->sol.p, r, z,
                                                    c->part_c.p, nsp, c->part_a.p, c->part_b.p, c->pcg_state.p, it);
-            k_pcg_upd2<V><<<nu, TPB, 0, c->s>>>(nv, c->scal.p, z, p, c->qc.p, c->part_a.p, c->part_b.p, nu,
-                                                c->pcg_state.p, it, tol2);
+            k_pcg_upd2<V><<<nu, TPB, 0, c->s>>>(nv, c->scal.p, z, p, c->part_a.p, c->part_b.p, nu, c->pcg_state.p,
+                                                it, tol2);
             STCHK(fipc_tm_end(c, TM_PCGVEC));
         }
         HIPCHK(hipGetLastError());
@@ -1542,7 +1701,7 @@ extern "C" int fipc_batch_constraint(int type, int n, const double* x, const dou
     if (type < 0 || type >= NTYPES || n < 0) return FIPC_ERR_ARG;
     if (n == 0) return FIPC_OK;
     int K = type == 0 ? 2 : (type == 1 ? 3 : 4);
-    int W = K, N = 3 * K, G = K == 2 ? 2 : 4;
+    int W = K, N = 3 * K;
     // the kernel reads rows of vertex ids: make identity rows over the packed points
     std::vector<int> rows((size_t)n * W);
     for (int r = 0; r < n; r++)
@@ -1559,18 +1718,10 @@ extern "C" int fipc_batch_constraint(int type, int n, const double* x, const dou
     HIPCHK(hipMemcpy(dx, x, npts * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dx0, x0 ? x0 : x, npts * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(drows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
-    int nb = nblk((long)n * G);
-#define LAUNCH_B(KK) k_contact<KK><<<nb, TPB>>>(n, drows, dx, dx0, dHat2, kappa, dH, nullptr, nullptr, dE, dg, project, 0, nullptr)
-    switch (type) {
-    case 0: LAUNCH_B(0); break;
-    case 1: LAUNCH_B(1); break;
-    case 2: LAUNCH_B(2); break;
-    case 3: LAUNCH_B(3); break;
-    case 4: LAUNCH_B(4); break;
-    case 5: LAUNCH_B(5); break;
-    default: LAUNCH_B(6); break;
-    }
-#undef LAUNCH_B
+    ContactBatch cb;
+    for (int t = 0; t < NTYPES; t++) { cb.rows[t] = drows; cb.H[t] = dH; cb.n[t] = t == type ? n : 0; }
+    int nb = contact_blocks(cb);
+    k_contact<<<nb, TPB>>>(cb, dx, dx0, dHat2, kappa, nullptr, nullptr, dE, dg, project, 0, nullptr, 20);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     if (E) HIPCHK(hipMemcpy(E, dE, (size_t)n * 8, hipMemcpyDeviceToHost));

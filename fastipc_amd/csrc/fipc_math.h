@@ -670,24 +670,20 @@ FI void tet_hessian_blocks(const NHFactor& f, const double (&Bm)[4][3], double s
         }
 #pragma unroll
         for (int c = a; c < 4; c++) {
-            // z_l[(c,j)] = bv[c][lb] U[j][la] for l = la*3+lb
-            double Zc[3][9];
+            // z_l[(c,j)] = bv[c][lb] U[j][la] for l = la*3+lb, so
+            // blk[i][j] = sum_la U[j][la] T[i][la], T[i][la] = sum_lb Y[i][la*3+lb] bv[c][lb]
+            double T[3][3];
 #pragma unroll
-            for (int j = 0; j < 3; j++)
+            for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int la = 0; la < 3; la++)
-#pragma unroll
-                    for (int lb = 0; lb < 3; lb++) Zc[j][la * 3 + lb] = bv[c][lb] * f.U[j * 3 + la];
+                    T[i][la] = Y[i][la * 3] * bv[c][0] + Y[i][la * 3 + 1] * bv[c][1] + Y[i][la * 3 + 2] * bv[c][2];
             double blk[9];
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    double s = 0.0;
-#pragma unroll
-                    for (int l = 0; l < 9; l++) s += Y[i][l] * Zc[j][l];
-                    blk[i * 3 + j] = s;
-                }
+                for (int j = 0; j < 3; j++)
+                    blk[i * 3 + j] = T[i][0] * f.U[j * 3] + T[i][1] * f.U[j * 3 + 1] + T[i][2] * f.U[j * 3 + 2];
             emit(a, c, blk);
         }
     }

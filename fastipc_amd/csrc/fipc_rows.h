@@ -12,7 +12,9 @@ template <int KIND>
 struct KindInfo {
     static constexpr int K = kind_points(KIND);
     static constexpr int N = 3 * K;
-    static constexpr int G = K == 2 ? 2 : 4;
+    // lanes per row of the contact assembly: one per Hessian row, rounded up
+    // to a power of two
+    static constexpr int G = N <= 8 ? 8 : 16;
 };
 
 template <int KIND>
