@@ -103,7 +103,7 @@ __global__ void k_grid_count(int n, BoxFn box, double inv, int* cnt, int* flag)
 // search in the scanned counts, so an item covering thousands of cells (a
 // long plate edge) does not serialise a wave.
 __global__ void k_grid_fill(int n, int cap, BoxFn box, double inv, unsigned M, const int* __restrict__ off,
-                            unsigned* ekey, int* eitem, unsigned long long* ecell, int* flag)
+                            unsigned* ekey, int* eitem, unsigned long long* ecell, int* bcount, int* flag)
 {
     int ne = off[n];
     if (ne > cap || ne < 0) {
@@ -127,27 +127,24 @@ __global__ void k_grid_fill(int n, int cap, BoxFn box, double inv, unsigned M, c
         int l = e - off[i];
         int a = r.lo[0] + l / (ny * nz), b = r.lo[1] + (l / nz) % ny, c = r.lo[2] + l % nz;
         unsigned flags = (a == r.lo[0] ? 1u : 0u) | (b == r.lo[1] ? 2u : 0u) | (c == r.lo[2] ? 4u : 0u);
-        ekey[e] = bucket_of(a, b, c, M);
+        unsigned k = bucket_of(a, b, c, M);
+        ekey[e] = k;
         eitem[e] = (int)((unsigned)i | (flags << 28));
         ecell[e] = cell_key(a, b, c);
+        atomicAdd(&bcount[k], 1);
     }
 }
-__global__ void k_grid_hist(const int* __restrict__ nep, int cap, const unsigned* ekey, int* bcount)
-{
-    int ne = *nep;
-    if (ne > cap) return;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x)
-        atomicAdd(&bcount[ekey[e]], 1);
-}
+// bstart holds the scanned bucket starts; each entry takes the next slot of
+// its bucket, which leaves bstart[k] at the end of bucket k (the start of
+// k + 1): bucket k is [k ? bstart[k - 1] : 0, bstart[k]) afterwards
 __global__ void k_grid_scatter(const int* __restrict__ nep, int cap, const unsigned* ekey, const int* eitem,
-                               const unsigned long long* ecell, const int* bstart, int* bfill, int* items,
-                               unsigned long long* cells)
+                               const unsigned long long* ecell, int* bstart, int* items, unsigned long long* cells)
 {
     int ne = *nep;
     if (ne > cap) return;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
         unsigned k = ekey[e];
-        int pos = bstart[k] + atomicAdd(&bfill[k], 1);
+        int pos = atomicAdd(&bstart[k], 1);
         items[pos] = eitem[e];
         cells[pos] = ecell[e];
     }
@@ -198,7 +195,7 @@ __global__ void __launch_bounds__(TPB) k_grid_query(int nq, const int* __restric
         int c = r.lo[2] + l % nz;
         unsigned long long key = cell_key(a, b, c);
         unsigned bk = bucket_of(a, b, c, M);
-        int e0 = bstart[bk], e1 = bstart[bk + 1];
+        int e0 = bk ? bstart[bk - 1] : 0, e1 = bstart[bk];
         for (int e = e0; e < e1; e++) {
             if (cells[e] != key) continue;
             unsigned it = (unsigned)items[e];
@@ -227,26 +224,25 @@ static int grid_build(fipc_ctx* c, int n, const BoxFn& box)
     Grid& g = c->grid;
     STCHK(fipc_tm_begin(c, TM_GRID));
     int cap = g.cap;
+    // buckets: a power of two at least the entry capacity
     unsigned M = 1024;
-    while (M < (unsigned)cap * 2u && M < (1u << 30)) M <<= 1;
+    while (M < (unsigned)cap && M < (1u << 30)) M <<= 1;
     g.M = M;
     STCHK(ensure(g.cnt, (size_t)n + 2));
     STCHK(ensure(g.ekey, cap)); STCHK(ensure(g.eitem, cap)); STCHK(ensure(g.items, cap));
     STCHK(ensure(g.ecell, cap)); STCHK(ensure(g.cells, cap));
-    STCHK(ensure(g.bstart, (size_t)M + 1)); STCHK(ensure(g.bfill, M));
+    STCHK(ensure(g.bstart, (size_t)M + 1));
     k_grid_count<<<nblk(n + 1), TPB, 0, c->s>>>(n, box, c->inv_dx, g.cnt.p, c->flag.p);
     HIPCHK(hipGetLastError());
     STCHK(scan_inplace(c, g.cnt.p, n + 1));
     const int* nep = g.cnt.p + n;
     int nb = stride_grid(cap);
-    k_grid_fill<<<nb, TPB, 0, c->s>>>(n, cap, box, c->inv_dx, M, g.cnt.p, g.ekey.p, g.eitem.p, g.ecell.p, c->flag.p);
     HIPCHK(hipMemsetAsync(g.bstart.p, 0, ((size_t)M + 1) * 4, c->s));
-    HIPCHK(hipMemsetAsync(g.bfill.p, 0, (size_t)M * 4, c->s));
-    k_grid_hist<<<nb, TPB, 0, c->s>>>(nep, cap, g.ekey.p, g.bstart.p);
+    k_grid_fill<<<nb, TPB, 0, c->s>>>(n, cap, box, c->inv_dx, M, g.cnt.p, g.ekey.p, g.eitem.p, g.ecell.p, g.bstart.p,
+                                      c->flag.p);
     HIPCHK(hipGetLastError());
     STCHK(scan_inplace(c, g.bstart.p, M + 1));
-    k_grid_scatter<<<nb, TPB, 0, c->s>>>(nep, cap, g.ekey.p, g.eitem.p, g.ecell.p, g.bstart.p, g.bfill.p, g.items.p,
-                                         g.cells.p);
+    k_grid_scatter<<<nb, TPB, 0, c->s>>>(nep, cap, g.ekey.p, g.eitem.p, g.ecell.p, g.bstart.p, g.items.p, g.cells.p);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_GRID));
     return FIPC_OK;

@@ -67,8 +67,7 @@ struct Grid {
     DVec<unsigned> ekey;   // bucket of each entry
     DVec<int> eitem;       // item | lowest-cell flags << 28
     DVec<unsigned long long> ecell, cells;  // exact cell key per entry / ordered by bucket
-    DVec<int> bstart;      // M+1 bucket offsets
-    DVec<int> bfill;       // M cursors
+    DVec<int> bstart;      // M+1 bucket offsets (bucket ends after the scatter)
     DVec<int> items;       // entries ordered by bucket
     DVec<int> qoff;        // query cell offsets (nq+1)
 };

@@ -1021,7 +1021,7 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->part_a); release(c->part_b); release(c->part_c); release(c->scal); release(c->pcg_state);
     release(c->epart); release(c->amin); release(c->flag);
     release(c->grid.cnt); release(c->grid.ekey); release(c->grid.eitem); release(c->grid.bstart);
-    release(c->grid.bfill); release(c->grid.items); release(c->grid.ecell); release(c->grid.cells);
+    release(c->grid.items); release(c->grid.ecell); release(c->grid.cells);
     release(c->grid.qoff); release(c->ccd_pairs[0]); release(c->ccd_pairs[1]);
     release(c->dprev); release(c->wsX); release(c->ckey); release(c->free_ids); release(c->cap_ids);
     release(c->cap_X); release(c->cap_T);
