@@ -70,6 +70,7 @@ def algorithmic_bytes(sim, precision):
     from fastipc_amd import _lib
     _lib.call("fipc_bsr_storage", sim.ctx, C.byref(nu), C.byref(nl))
     nu, nl = nu.value, nl.value
+    # FP32 Hessian: FP32 block values and FP32 iteration vectors p, q
     sa = 4 if precision == 1 else 8
     width = (2, 3, 4, 4, 4, 4, 4)
     contact = sum(n * ((3 * w) ** 2 * 8 + 4 * w) for n, w in zip(sim.counts, width))
@@ -78,7 +79,7 @@ def algorithmic_bytes(sim, precision):
         # lower block, both row-pointer arrays, p read, q written, contact rows
         # (dense H_r + indices); the transposed re-read of a stored block is
         # not compulsory traffic
-        "spmv": nu * (9 * sa + 4) + nl * 8 + 8 * (nv + 1) + 2 * 3 * nv * 8 + contact,
+        "spmv": nu * (9 * sa + 4) + nl * 8 + 8 * (nv + 1) + 2 * 3 * nv * sa + contact,
         # per tet: 4 indices, D_m^-1, vol, lambda, mu, 10 contribution
         # positions; per vertex x; the stored block values written once and
         # the gradient

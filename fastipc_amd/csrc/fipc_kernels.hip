@@ -897,8 +897,12 @@ __global__ void k_inversion(int nt, const int* tets, const double* x, const doub
     double al = 1.0;
     if (e < nt) {
         int a = tets[4 * e], b = tets[4 * e + 1], c = tets[4 * e + 2], d = tets[4 * e + 3];
-        al = tet_inversion_alpha(ldv(x, a), ldv(x, b), ldv(x, c), ldv(x, d), ldv(p, a), ldv(p, b), ldv(p, c),
-                                 ldv(p, d), 0.2);
+        v3 x1 = ldv(x, a), x2 = ldv(x, b), x3 = ldv(x, c), x4 = ldv(x, d);
+        v3 p1 = ldv(p, a), p2 = ldv(p, b), p3 = ldv(p, c), p4 = ldv(p, d);
+        double ca, cb, cc, cd;
+        tet_det_cubic(x1, x2, x3, x4, p1, p2, p3, p4, ca, cb, cc, cd);
+        // the root finder only for the tets whose bound may be below 1
+        if (!tet_inversion_clear(ca, cb, cc, cd, 0.2)) al = tet_inversion_alpha(x1, x2, x3, x4, p1, p2, p3, p4, 0.2);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) al = fmin(al, __shfl_xor(al, o, 64));

@@ -756,14 +756,33 @@ FI double cubic_root(double a, double b, double c, double d, double tol)
     }
     return t;
 }
-// inversion-free step bound of one tet (math_tools.py:160-181)
-FI double tet_inversion_alpha(v3 x1, v3 x2, v3 x3, v3 x4, v3 p1, v3 p2, v3 p3, v3 p4, double eta)
+// coefficients of det(x + t p) = a t^3 + b t^2 + c t + d of a tet's edge
+// matrix (math_tools.py:160-181)
+FI void tet_det_cubic(v3 x1, v3 x2, v3 x3, v3 x4, v3 p1, v3 p2, v3 p3, v3 p4, double& a, double& b, double& c,
+                      double& d)
 {
     v3 v01 = x2 - x1, v02 = x3 - x1, v03 = x4 - x1, p01 = p2 - p1, p02 = p3 - p1, p03 = p4 - p1;
     v3 pp = cross(p01, p02), s = cross(v01, p02) + cross(p01, v02), c0 = cross(v01, v02);
-    double a = dot(p03, pp), b = dot(v03, pp) + dot(p03, s), c = dot(p03, c0) + dot(v03, s), d = dot(v03, c0);
+    a = dot(p03, pp); b = dot(v03, pp) + dot(p03, s); c = dot(p03, c0) + dot(v03, s); d = dot(v03, c0);
+}
+// inversion-free step bound of one tet (math_tools.py:160-181)
+FI double tet_inversion_alpha(v3 x1, v3 x2, v3 x3, v3 x4, v3 p1, v3 p2, v3 p3, v3 p4, double eta)
+{
+    double a, b, c, d;
+    tet_det_cubic(x1, x2, x3, x4, p1, p2, p3, p4, a, b, c, d);
     double r = cubic_root(a, b, c, d, 1e-8);
     return r < 0.0 ? 1.0 : r * (1.0 - eta);
+}
+// true when the tet's bound is certainly >= 1, so that it cannot lower a
+// minimum that starts at 1: the bound is r (1 - eta) for the smallest
+// positive root r, which matters only for r < T = 1 / (1 - eta); with d > 0
+// and 2 (|a| T^3 + |b| T^2 + |c| T) < d, det(x + t p) > d / 2 on [0, T], so
+// there is no real root there and no complex one close enough to the real
+// axis to pass the root finder's tolerance
+FI bool tet_inversion_clear(double a, double b, double c, double d, double eta)
+{
+    double T = 1.0 / (1.0 - eta);
+    return d > 0.0 && 2.0 * (fabs(a) * T * T * T + fabs(b) * T * T + fabs(c) * T) < d;
 }
 
 // ---------------------------------------------------------------------------

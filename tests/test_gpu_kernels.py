@@ -129,3 +129,27 @@ def test_ccd_matches_oracle(kind):
     assert (ref < 1).sum() > 100
     assert np.abs(toc - ref).max() <= 1e-6
     assert np.all(toc <= ref + 1e-12)
+
+
+def test_inversion_bound_matches_oracle():
+    """The inversion part of the line-search filter (NEWTON_IPC.py:443-453)
+    equals the oracle's minimum over every tet, for displacements that invert
+    some tets and for small ones that invert none."""
+    from fastipc_amd import scene as S
+    from fastipc_amd.newton import NewtonIPC
+    sc = S.two_cubes(4)
+    sim = NewtonIPC(sc)
+    sim.compute_xn_and_xTilde(0.0)
+    rng = np.random.default_rng(5)
+    x = sim.x
+    for scale in (0.3, 0.05, 1e-3):
+        p = rng.normal(size=x.shape) * scale * sc.bbox_diag
+        sim.set_vector(1, p)
+        a = C.c_double()
+        parts = np.zeros(3)
+        _lib.call("fipc_ccd_alpha", sim.ctx, sc.eta, 1, C.byref(a), _lib.dptr(parts))
+        ref = 1.0
+        for t in sc.tets:
+            ref = min(ref, O.tet_inversion_alpha([x[i] for i in t], [p[i] for i in t], 0.2))
+        assert parts[2] == pytest.approx(ref, rel=1e-9, abs=1e-12), (scale, parts[2], ref)
+        print(f"scale {scale}: inversion bound {parts[2]}")

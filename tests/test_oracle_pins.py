@@ -148,6 +148,43 @@ def test_cubic_root_vs_reference_and_numpy():
                     assert o == pytest.approx(min(pos), rel=1e-5, abs=1e-9)
 
 
+def _det_cubic(xs, ps):
+    """Coefficients of det[x2 - x1 + t (p2 - p1), ...] by sampling the cubic
+    at four t and solving the Vandermonde system (independent of the kernel's
+    cross-product expansion)."""
+    ts = np.array([0.0, 0.5, 1.0, 2.0])
+    def det(t):
+        y = [xs[i] + t * ps[i] for i in range(4)]
+        return np.linalg.det(np.stack([y[1] - y[0], y[2] - y[0], y[3] - y[0]]))
+    v = np.array([det(t) for t in ts])
+    a, b, c, d = np.linalg.solve(np.vander(ts, 4), v)
+    return a, b, c, d
+
+
+def test_inversion_certificate_is_sound():
+    """Tets that the inversion kernel skips (2 (|a| T^3 + |b| T^2 + |c| T) < d,
+    T = 1 / (1 - eta)) have an oracle bound of at least 1, so skipping them
+    cannot change the minimum, which starts at 1."""
+    rng = np.random.default_rng(11)
+    eta, T = 0.2, 1.25
+    cleared = kept = 0
+    for trial in range(3000):
+        xs = [np.zeros(3), np.array([1.0, 0, 0]), np.array([0, 1.0, 0]), np.array([0, 0, 1.0])]
+        xs = [x + rng.normal(size=3) * 0.1 for x in xs]
+        scale = 10.0 ** rng.uniform(-3, 0.3)
+        ps = [rng.normal(size=3) * scale for _ in range(4)]
+        a, b, c, d = _det_cubic(xs, ps)
+        clear = d > 0 and 2 * (abs(a) * T ** 3 + abs(b) * T ** 2 + abs(c) * T) < d
+        al = O.tet_inversion_alpha(xs, ps, eta)
+        if clear:
+            cleared += 1
+            assert al >= 1.0, (trial, a, b, c, d, al)
+        else:
+            kept += 1
+    # both branches are exercised, and small steps are cleared
+    assert cleared > 300 and kept > 300, (cleared, kept)
+
+
 @pytest.mark.parametrize("name,fn", [("pt", "pt_ccd"), ("ee", "ee_ccd")])
 def test_ccd_vs_reference(name, fn):
     P, D, d2, T = (REF[f"ccd_{name}_{s}"] for s in ("pts", "disp", "d2", "toc"))
