@@ -219,9 +219,13 @@ static int scan_inplace(fipc_ctx* c, int* d, int n)
 
 // Build the grid over n items without waiting for the host: buffers are
 // sized by the capacity, overflow is flagged in flag[0] (see GF_RETRY).
-static int grid_build(fipc_ctx* c, int n, const BoxFn& box)
+static int grid_build(fipc_ctx* c, int slot, int n, const BoxFn& box)
 {
-    Grid& g = c->grid;
+    Grid& g = c->grid[slot];
+    // the same boxes at the same x: the grid is already there
+    if (g.reusable && !box.sol && g.x_version == c->x_version && g.kind == box.kind && g.n == n && g.infl == box.infl)
+        return FIPC_OK;
+    g.reusable = false;
     STCHK(fipc_tm_begin(c, TM_GRID));
     int cap = g.cap;
     // buckets: a power of two at least the entry capacity
@@ -244,16 +248,23 @@ static int grid_build(fipc_ctx* c, int n, const BoxFn& box)
     STCHK(scan_inplace(c, g.bstart.p, M + 1));
     k_grid_scatter<<<nb, TPB, 0, c->s>>>(nep, cap, g.ekey.p, g.eitem.p, g.ecell.p, g.bstart.p, g.items.p, g.cells.p);
     HIPCHK(hipGetLastError());
+    // reusable once built; an overflow found after the operation's
+    // synchronisation clears this again (bp_check)
+    g.reusable = box.sol == nullptr;
+    g.x_version = c->x_version;
+    g.kind = box.kind;
+    g.n = n;
+    g.infl = box.infl;
     STCHK(fipc_tm_end(c, TM_GRID));
     return FIPC_OK;
 }
 
 // query the last grid with nq boxes; no host synchronisation
 template <class Visit>
-static int run_query(fipc_ctx* c, int nq, const BoxFn& qbox, Visit visit, int timer)
+static int run_query(fipc_ctx* c, int slot, int nq, const BoxFn& qbox, Visit visit, int timer)
 {
     if (nq == 0) return FIPC_OK;
-    Grid& g = c->grid;
+    Grid& g = c->grid[slot];
     STCHK(ensure(g.qoff, (size_t)nq + 2));
     STCHK(fipc_tm_begin(c, timer));
     k_query_cells<<<nblk(nq + 1), TPB, 0, c->s>>>(nq, qbox, c->inv_dx, g.qoff.p, c->flag.p);
@@ -275,13 +286,14 @@ static int bp_begin(fipc_ctx* c)
 // after the operation's synchronisation: error, retry (capacity grown) or done
 static int bp_check(fipc_ctx* c, const int* hflag, bool* retry)
 {
-    if (hflag[0] & GF_HUGE) return FIPC_ERR_CAPACITY;
     *retry = false;
+    if (hflag[0] & (GF_HUGE | GF_RETRY)) c->grid[0].reusable = c->grid[1].reusable = false;
+    if (hflag[0] & GF_HUGE) return FIPC_ERR_CAPACITY;
     if (hflag[0] & GF_RETRY) {
         long need = (long)hflag[2];
         long cap = need + need / 4 + 4096;
         if (cap > (1L << 30)) return FIPC_ERR_CAPACITY;
-        c->grid.cap = (int)cap;
+        for (Grid& g : c->grid) g.cap = (int)std::max<long>(g.cap, cap);
         *retry = true;
     }
     return FIPC_OK;
@@ -433,16 +445,16 @@ static int find_once(fipc_ctx* c, int* counts, int* hflag)
     sink.kpe = c->keys_pe.p;
     // PT: points inserted with +-dHat, triangles query their box
     BoxFn pts{c->x.p, nullptr, c->bp.p, 0, c->dHat};
-    STCHK(grid_build(c, c->nbp, pts));
+    STCHK(grid_build(c, 0, c->nbp, pts));
     BoxFn tris{c->x.p, nullptr, c->bt.p, 2, 0.0};
     VisitPT vpt{c->x.p, c->bp.p, c->bt.p, c->dHat, c->dHat2, sink};
-    STCHK(run_query(c, c->nbt, tris, vpt, TM_FIND));
+    STCHK(run_query(c, 0, c->nbt, tris, vpt, TM_FIND));
     // EE: edges inserted with their box, queried with box +- dHat
     BoxFn edges{c->x.p, nullptr, c->be.p, 1, 0.0};
-    STCHK(grid_build(c, c->nbe, edges));
+    STCHK(grid_build(c, 1, c->nbe, edges));
     BoxFn qedges{c->x.p, nullptr, c->be.p, 1, c->dHat};
     VisitEE vee{c->x.p, c->x0.p, c->be.p, c->dHat, c->dHat2, sink};
-    STCHK(run_query(c, c->nbe, qedges, vee, TM_FIND));
+    STCHK(run_query(c, 1, c->nbe, qedges, vee, TM_FIND));
     HIPCHK(hipMemcpyAsync(counts, c->counters.p, NTYPES * 4, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipMemcpyAsync(hflag, c->flag.p, 12, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
@@ -648,20 +660,20 @@ extern "C" int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* al
         for (int i = 0; i < 3; i++) memcpy(&init[i], &one, 8);
         HIPCHK(hipMemcpyAsync(c->amin.p, init, 24, hipMemcpyHostToDevice, c->s));
         BoxFn pts{c->x.p, c->sol.p, c->bp.p, 0, c->dHat};
-        STCHK(grid_build(c, c->nbp, pts));
+        STCHK(grid_build(c, 0, c->nbp, pts));
         BoxFn tris{c->x.p, c->sol.p, c->bt.p, 2, 0.0};
         CandCCDPT vpt{c->x.p, c->sol.p, c->bp.p, c->bt.p, c->dHat, eta, PairSink{pc, cap, c->ccd_pairs[0].p}};
-        STCHK(run_query(c, c->nbt, tris, vpt, TM_CCD));
+        STCHK(run_query(c, 0, c->nbt, tris, vpt, TM_CCD));
         STCHK(fipc_tm_begin(c, TM_CCD));
         k_ccd_pt<<<stride_grid(cap), TPB, 0, c->s>>>(pc, cap, c->ccd_pairs[0].p, c->x.p, c->sol.p, c->bp.p, c->bt.p, eta,
                                                      c->amin.p);
         HIPCHK(hipGetLastError());
         STCHK(fipc_tm_end(c, TM_CCD));
         BoxFn edges{c->x.p, c->sol.p, c->be.p, 1, 0.0};
-        STCHK(grid_build(c, c->nbe, edges));
+        STCHK(grid_build(c, 1, c->nbe, edges));
         BoxFn qedges{c->x.p, c->sol.p, c->be.p, 1, c->dHat};
         CandCCDEE vee{c->x.p, c->sol.p, c->be.p, c->dHat, eta, PairSink{pc + 1, cap, c->ccd_pairs[1].p}};
-        STCHK(run_query(c, c->nbe, qedges, vee, TM_CCD));
+        STCHK(run_query(c, 1, c->nbe, qedges, vee, TM_CCD));
         STCHK(fipc_tm_begin(c, TM_CCD));
         k_ccd_ee<<<stride_grid(cap), TPB, 0, c->s>>>(pc + 1, cap, c->ccd_pairs[1].p, c->x.p, c->sol.p, c->be.p, eta,
                                                      c->amin.p + 1);
@@ -710,11 +722,14 @@ extern "C" int fipc_check_collision(fipc_ctx* c, int* hit)
     for (int attempt = 0;; attempt++) {
         if (attempt == 6) return FIPC_ERR_CAPACITY;
         STCHK(bp_begin(c));
-        BoxFn edges{c->x.p, nullptr, c->be.p, 1, c->dHat};
-        STCHK(grid_build(c, c->nbe, edges));
-        BoxFn tris{c->x.p, nullptr, c->bt.p, 2, 0.0};
+        // the edge grid of the constraint search (boxes without margin), so
+        // either reuses the other's at the same x; the margin d-hat moves to
+        // the triangle queries, which pairs the same boxes
+        BoxFn edges{c->x.p, nullptr, c->be.p, 1, 0.0};
+        STCHK(grid_build(c, 1, c->nbe, edges));
+        BoxFn tris{c->x.p, nullptr, c->bt.p, 2, c->dHat};
         VisitCollide v{c->x.p, c->be.p, c->bt.p, c->flag.p + 1};
-        STCHK(run_query(c, c->nbt, tris, v, TM_FIND));
+        STCHK(run_query(c, 1, c->nbt, tris, v, TM_FIND));
         int hflag[3];
         HIPCHK(hipMemcpyAsync(hflag, c->flag.p, 12, hipMemcpyDeviceToHost, c->s));
         HIPCHK(hipStreamSynchronize(c->s));

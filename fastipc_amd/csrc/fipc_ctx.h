@@ -61,7 +61,7 @@ static inline void release(DVec<T>& v)
 #define GF_RETRY 1
 #define GF_HUGE 2
 struct Grid {
-    unsigned M = 0;        // bucket count (power of two, >= 2 cap)
+    unsigned M = 0;        // bucket count (power of two, >= cap)
     int cap = 0;           // entry capacity; grows when a build overflows
     DVec<int> cnt;         // per item cell count, scanned in place (n+1); cnt[n] = entries
     DVec<unsigned> ekey;   // bucket of each entry
@@ -70,6 +70,13 @@ struct Grid {
     DVec<int> bstart;      // M+1 bucket offsets (bucket ends after the scatter)
     DVec<int> items;       // entries ordered by bucket
     DVec<int> qoff;        // query cell offsets (nq+1)
+    // what the grid holds, for reuse by a later operation at the same x: the
+    // x version, primitive kind, count and inflation (swept grids are never
+    // reused)
+    bool reusable = false;
+    unsigned long long x_version = 0;
+    int kind = -1, n = -1;
+    double infl = 0.0;
 };
 
 // HIP-event timing of single kernels: pairs are recorded on the context
@@ -135,7 +142,8 @@ struct fipc_ctx {
     int ccd_cap = 0;
     DVec<int2> ccd_pairs[2];
     // broad phase grids
-    Grid grid;
+    Grid grid[2];                         // [0] boundary points, [1] boundary edges
+    unsigned long long x_version = 1;     // incremented by every write of x
     // learned warm start
     DVec<double> dprev;    // previous step's correction x_{n} - xTilde_{n-1} (0 on Dirichlet vertices)
     DVec<float> wsX;       // features, nv x 16

@@ -538,7 +538,7 @@ __global__ void k_precond(int nv, const double* __restrict__ val, const int* __r
 // ---------------------------------------------------------------------------
 // SpMV: q += A p  (BSR part) and q += sum_rows H_r p_r (contact part)
 // ---------------------------------------------------------------------------
-#define SPMV_G 8
+#define SPMV_G 4
 // One launch computes A p: blocks [0, nb_c) run the contact rows of every
 // type (first, so that they overlap the BSR rows instead of trailing them),
 // the nb_bsr blocks after them the BSR rows (nb_c is a multiple of 8, so the
@@ -1024,9 +1024,11 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->Minvf); release(c->rf); release(c->zf); release(c->pf); release(c->qf);
     release(c->part_a); release(c->part_b); release(c->part_c); release(c->scal); release(c->pcg_state);
     release(c->epart); release(c->amin); release(c->flag);
-    release(c->grid.cnt); release(c->grid.ekey); release(c->grid.eitem); release(c->grid.bstart);
-    release(c->grid.items); release(c->grid.ecell); release(c->grid.cells);
-    release(c->grid.qoff); release(c->ccd_pairs[0]); release(c->ccd_pairs[1]);
+    for (Grid& g : c->grid) {
+        release(g.cnt); release(g.ekey); release(g.eitem); release(g.bstart);
+        release(g.items); release(g.ecell); release(g.cells); release(g.qoff);
+    }
+    release(c->ccd_pairs[0]); release(c->ccd_pairs[1]);
     release(c->dprev); release(c->wsX); release(c->ckey); release(c->free_ids); release(c->cap_ids);
     release(c->cap_X); release(c->cap_T);
     (void)fipc_tm_fold(c);
@@ -1148,6 +1150,7 @@ extern "C" int fipc_upload_mesh(fipc_ctx* c, int nv, const double* x0, const dou
     STCHK(ensure(c->r, n3)); STCHK(ensure(c->z, n3)); STCHK(ensure(c->pp, n3)); STCHK(ensure(c->q, n3)); STCHK(ensure(c->qc, n3));
     STCHK(ensure(c->cdiag, (size_t)nv * 9)); STCHK(ensure(c->Minv, (size_t)nv * 9));
     HIPCHK(hipMemcpyAsync(c->x.p, x0, n3 * 8, hipMemcpyHostToDevice, c->s));
+    c->x_version++;
     HIPCHK(hipMemcpyAsync(c->xt.p, x0, n3 * 8, hipMemcpyHostToDevice, c->s));
     HIPCHK(hipMemsetAsync(c->v.p, 0, n3 * 8, c->s));
     STCHK(ensure(c->dprev, n3));
@@ -1164,7 +1167,8 @@ extern "C" int fipc_upload_mesh(fipc_ctx* c, int nv, const double* x0, const dou
     STCHK(ensure(c->epart, nblk(nmax) * 4 + 64));
     STCHK(ensure(c->amin, 4)); STCHK(ensure(c->flag, 4)); STCHK(ensure(c->counters, 16)); STCHK(ensure(c->nuniq, 4));
     // broad-phase capacities start from the primitive counts and grow on demand
-    if (c->grid.cap == 0) c->grid.cap = 4 * std::max(nbp, std::max(nbe, nbt)) + 4096;
+    for (Grid& g : c->grid)
+        if (g.cap == 0) g.cap = 4 * std::max(nbp, std::max(nbe, nbt)) + 4096;
     if (c->ccd_cap == 0) c->ccd_cap = 65536;
     for (int t = 0; t < NTYPES; t++) {
         if (c->cap[t] == 0) c->cap[t] = 1024;
@@ -1188,7 +1192,10 @@ extern "C" int fipc_set_state(fipc_ctx* c, const double* x, const double* v)
 {
     if (!c || !c->nv) return FIPC_ERR_STATE;
     size_t n = (size_t)c->nv * 3 * 8;
-    if (x) HIPCHK(hipMemcpyAsync(c->x.p, x, n, hipMemcpyHostToDevice, c->s));
+    if (x) {
+        HIPCHK(hipMemcpyAsync(c->x.p, x, n, hipMemcpyHostToDevice, c->s));
+        c->x_version++;
+    }
     if (v) HIPCHK(hipMemcpyAsync(c->v.p, v, n, hipMemcpyHostToDevice, c->s));
     // a new (x, v) state has no previous step
     if (v) HIPCHK(hipMemsetAsync(c->dprev.p, 0, n, c->s));
@@ -1231,6 +1238,7 @@ extern "C" int fipc_set_vector(fipc_ctx* c, int which, const double* in)
     DVec<double>* d = vec_of(c, which);
     if (!d || !in) return FIPC_ERR_ARG;
     HIPCHK(hipMemcpyAsync(d->p, in, (size_t)c->nv * 24, hipMemcpyHostToDevice, c->s));
+    if (d == &c->x) c->x_version++;
     HIPCHK(hipStreamSynchronize(c->s));
     return FIPC_OK;
 }
@@ -1264,6 +1272,7 @@ extern "C" int fipc_apply_sol(fipc_ctx* c, double alpha)
     if (!c || !c->nv) return FIPC_ERR_STATE;
     k_apply<<<nblk(c->nv * 3), TPB, 0, c->s>>>(c->nv * 3, c->xprev.p, c->sol.p, c->x.p, alpha);
     HIPCHK(hipGetLastError());
+    c->x_version++;
     return FIPC_OK;
 }
 extern "C" int fipc_residual(fipc_ctx* c, double* res)
