@@ -214,6 +214,8 @@ __global__ void k_gather_blocks(int nnzb, const int* __restrict__ bcptr, const d
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nnzb) return;
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // unrolled so that several records are in flight (same summation order)
+#pragma unroll 4
     for (int t = bcptr[b]; t < bcptr[b + 1]; t++) {
         const double* h = tetH + (size_t)t * 9;
 #pragma unroll
@@ -246,6 +248,7 @@ __global__ void k_gather_rhs(int nv, const int* __restrict__ vcptr, const int* _
     }
     double mi = m[i];
     double r0 = -mi * (x[3 * i] - xt[3 * i]), r1 = -mi * (x[3 * i + 1] - xt[3 * i + 1]), r2 = -mi * (x[3 * i + 2] - xt[3 * i + 2]);
+#pragma unroll 4
     for (int t = vcptr[i]; t < vcptr[i + 1]; t++) {
         const double* g = tetg + (size_t)vcent[t] * 3;
         r0 -= g[0]; r1 -= g[1]; r2 -= g[2];
@@ -640,6 +643,7 @@ __global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ ro
             // the row's stored blocks (i, j), j >= i, then the transposes of
             // the blocks (j, i), j < i, stored in the rows above
             double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll 2
             for (int k = lane; k < n; k += SPMV_G) {
                 bool tr = k >= nu;
                 int b, c;
@@ -889,8 +893,9 @@ __global__ void k_min_dist(int n, const int* rows, const double* x, unsigned lon
 }
 
 // inversion filter (NEWTON_IPC.py:443-453): the minimum over the tets, folded
-// into *amin (a non-negative double's bit pattern) with one atomic per
-// wavefront
+// into *amin (a non-negative double's bit pattern, 1 on entry) with one
+// atomic per wavefront whose minimum is below 1 (same-address atomics
+// serialise, so the waves that cannot lower the minimum issue none)
 __global__ void k_inversion(int nt, const int* tets, const double* x, const double* p, unsigned long long* amin)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -906,7 +911,7 @@ __global__ void k_inversion(int nt, const int* tets, const double* x, const doub
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) al = fmin(al, __shfl_xor(al, o, 64));
-    if ((threadIdx.x & 63) == 0) atomicMin(amin, (unsigned long long)__double_as_longlong(al));
+    if ((threadIdx.x & 63) == 0 && al < 1.0) atomicMin(amin, (unsigned long long)__double_as_longlong(al));
 }
 
 // batched tet evaluation for tests
