@@ -43,6 +43,7 @@ _SIGS = {
     "fipc_set_state": [_VP, _P, _P],
     "fipc_get_state": [_VP, _P, _P],
     "fipc_begin_step": [_VP, _P],
+    "fipc_begin_step_rows": [_VP, C.c_int, _IP, _P],
     "fipc_end_step": [_VP],
     "fipc_find_constraints": [_VP, _IP],
     "fipc_get_constraints": [_VP, C.c_int, _IP],

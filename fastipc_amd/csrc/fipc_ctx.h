@@ -132,6 +132,9 @@ struct fipc_ctx {
     // FP32 Hessian: the refinement rounds iterate on FP32 copies of the
     // preconditioner and of r, z, p, q (qc and the solution stay FP64)
     DVec<float> Minvf, rf, zf, pf, qf;
+    // prescribed rows of the step (fipc_begin_step_rows)
+    DVec<int> trow;
+    DVec<double> tval;
     DVec<double> part_a, part_b, part_c, scal;
     DVec<int> pcg_state;   // [0] done_at, [1] iterations
     // reductions for energy / residual / ccd

@@ -1027,6 +1027,7 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->keys_pp); release(c->keys_pe); release(c->keys_tmp); release(c->counters); release(c->cub_tmp);
     release(c->nuniq); release(c->cdiag); release(c->Minv); release(c->r); release(c->z); release(c->pp); release(c->q); release(c->qc);
     release(c->Minvf); release(c->rf); release(c->zf); release(c->pf); release(c->qf);
+    release(c->trow); release(c->tval);
     release(c->part_a); release(c->part_b); release(c->part_c); release(c->scal); release(c->pcg_state);
     release(c->epart); release(c->amin); release(c->flag);
     for (Grid& g : c->grid) {
@@ -1253,6 +1254,29 @@ extern "C" int fipc_begin_step(fipc_ctx* c, const double* target)
     if (!c || !c->nv) return FIPC_ERR_STATE;
     if (target) HIPCHK(hipMemcpyAsync(c->target.p, target, (size_t)c->nv * 24, hipMemcpyHostToDevice, c->s));
     else HIPCHK(hipMemcpyAsync(c->target.p, c->x.p, (size_t)c->nv * 24, hipMemcpyDeviceToDevice, c->s));
+    k_begin_step<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->x.p, c->v.p, c->xn.p, c->xt.p, c->dmask.p, c->target.p, c->dt, c->gravity);
+    HIPCHK(hipGetLastError());
+    return FIPC_OK;
+}
+// prescribed positions of the listed rows (Dirichlet vertices) into target
+__global__ void k_scatter_rows(int n, const int* __restrict__ rows, const double* __restrict__ val, double* target)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int r = rows[i];
+    target[3 * r] = val[3 * i]; target[3 * r + 1] = val[3 * i + 1]; target[3 * r + 2] = val[3 * i + 2];
+}
+extern "C" int fipc_begin_step_rows(fipc_ctx* c, int n, const int* rows, const double* target)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    if (n < 0 || (n && (!rows || !target))) return FIPC_ERR_ARG;
+    for (int i = 0; i < n; i++)
+        if (rows[i] < 0 || rows[i] >= c->nv) return FIPC_ERR_ARG;
+    if (n) {
+        STCHK(upload(c->trow, rows, n, c->s));
+        STCHK(upload(c->tval, target, (size_t)n * 3, c->s));
+        k_scatter_rows<<<nblk(n), TPB, 0, c->s>>>(n, c->trow.p, c->tval.p, c->target.p);
+    }
     k_begin_step<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->x.p, c->v.p, c->xn.p, c->xt.p, c->dmask.p, c->target.p, c->dt, c->gravity);
     HIPCHK(hipGetLastError());
     return FIPC_OK;

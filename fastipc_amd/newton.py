@@ -139,8 +139,8 @@ class NewtonIPC:
     # ------------------------------------------------- reference step functions
     def compute_xn_and_xTilde(self, t: float):
         """compute_xn_and_xTilde + move_nodes (NEWTON_IPC.py:497-511)."""
-        target = _lib.f64(self.scene.dirichlet(t))
-        _lib.call("fipc_begin_step", self.ctx, _lib.dptr(target))
+        rows, target = self.scene.dirichlet_rows(t)
+        _lib.call("fipc_begin_step_rows", self.ctx, len(rows), _lib.iptr(rows), _lib.dptr(target))
         self._invalidate(x_changed=False)  # xTilde changed: the energy, not the constraints
 
     def find_constraints(self):

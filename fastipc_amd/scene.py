@@ -164,6 +164,9 @@ class Scene:
     gravity: float = -9.8         # applied on axis 1 (NEWTON_IPC.py:502)
     steps: int = 1
     dirichlet_fn: Optional[Callable[[float], np.ndarray]] = None
+    # optional: the prescribed positions of the Dirichlet rows only, in the
+    # order of np.flatnonzero(dirichlet_mask), shape (n_fixed, 3)
+    dirichlet_rows_fn: Optional[Callable[[float], np.ndarray]] = None
     boundary: tuple = None        # (points, edges, triangles)
     dhat_rel: float = 1e-3        # dhat = dhat_rel * bbox diagonal
     tol_rel: float = 1e-2         # Newton stop: max|p| < tol_rel * dt * bbox diagonal
@@ -199,6 +202,21 @@ class Scene:
         if self.dirichlet_fn is None:
             return self.x.copy()
         return self.dirichlet_fn(t)
+
+    def dirichlet_rows(self, t: float):
+        """(rows, positions): the Dirichlet vertices (int32, ascending) and
+        their prescribed positions at time ``t``, shape (len(rows), 3)."""
+        mask = self.dirichlet_mask
+        key = (id(mask), int(mask.sum()))
+        if getattr(self, "_drows_key", None) != key:
+            self._drows = np.flatnonzero(mask).astype(np.int32)
+            self._drows_key = key
+        rows = self._drows
+        if self.dirichlet_rows_fn is not None:
+            return rows, np.ascontiguousarray(self.dirichlet_rows_fn(t), np.float64)
+        if self.dirichlet_fn is None:
+            return rows, np.ascontiguousarray(self.x[rows], np.float64)
+        return rows, np.ascontiguousarray(self.dirichlet_fn(t)[rows], np.float64)
 
     def lame(self):
         E, nu = self.youngs, self.poisson
@@ -341,18 +359,29 @@ def compressed_slab(nx: int = 50, ny: int = 80, nz: int = 50, steps: int = 200,
     speed = 0.5 * close_rate * Hh
     amp = math.radians(twist_deg)
 
-    def fn(t):
-        out = x0.copy()
+    def plates(t):
+        """the 16 plate vertices (bottom then top) at time t"""
         th = amp * math.sin(omega * t)
+        out = []
         for first, sgn in ((ib, 1.0), (it, -1.0)):
             R = rotation_matrix([0, 1, 0], sgn * th)
             p = x0[first:first + 8] - [cx, 0.0, cz]
             q = p @ R.T + [cx, 0.0, cz]
             q[:, 1] += sgn * speed * t
-            out[first:first + 8] = q
+            out.append(q)
+        return np.concatenate(out)
+
+    def fn(t):
+        out = x0.copy()
+        q = plates(t)
+        out[ib:ib + 8] = q[:8]
+        out[it:it + 8] = q[8:]
         return out
 
     scene.dirichlet_fn = fn
+    # the fixed vertices are exactly the plates' (bottom block first)
+    assert np.array_equal(np.flatnonzero(scene.dirichlet_mask), np.r_[ib:ib + 8, it:it + 8])
+    scene.dirichlet_rows_fn = plates
     return scene
 
 

@@ -153,6 +153,9 @@ int fipc_get_state(fipc_ctx* c, double* x, double* v);
  * (target: nv*3, only Dirichlet rows are read).  x of Dirichlet vertices is
  * left where it is; the solve prescribes their increment xTilde - x. */
 int fipc_begin_step(fipc_ctx* c, const double* target);
+/* the same with the prescribed positions of the listed rows only (the
+ * Dirichlet vertices; rows[n], target[n][3]); the other rows are not used */
+int fipc_begin_step_rows(fipc_ctx* c, int n, const int* rows, const double* target);
 /* compute_v (NEWTON_IPC.py:799-802) */
 int fipc_end_step(fipc_ctx* c);
 /* find_constraints (NEWTON_IPC.py:337-357): counts[7] after deduplication */

@@ -153,3 +153,23 @@ def test_inversion_bound_matches_oracle():
             ref = min(ref, O.tet_inversion_alpha([x[i] for i in t], [p[i] for i in t], 0.2))
         assert parts[2] == pytest.approx(ref, rel=1e-9, abs=1e-12), (scale, parts[2], ref)
         print(f"scale {scale}: inversion bound {parts[2]}")
+
+
+def test_begin_step_rows_matches_full_target():
+    """fipc_begin_step_rows (Dirichlet rows only) gives the same xTilde as
+    fipc_begin_step with the full target array; bad rows are rejected."""
+    from fastipc_amd import scene as S
+    from fastipc_amd.newton import NewtonIPC
+    sc = S.compressed_slab(6, 8, 6)
+    a, b = NewtonIPC(sc), NewtonIPC(sc)
+    for sim in (a, b):
+        sim.step()
+    t = 0.37
+    full = np.ascontiguousarray(sc.dirichlet(t), np.float64)
+    _lib.call("fipc_begin_step", a.ctx, _lib.dptr(full))
+    rows, xyz = sc.dirichlet_rows(t)
+    _lib.call("fipc_begin_step_rows", b.ctx, len(rows), _lib.iptr(rows), _lib.dptr(xyz))
+    assert np.array_equal(a.vector(3), b.vector(3))
+    bad = np.array([a.nv], np.int32)
+    with pytest.raises(_lib.FipcError):
+        _lib.call("fipc_begin_step_rows", b.ctx, 1, _lib.iptr(bad), _lib.dptr(xyz[:1].copy()))

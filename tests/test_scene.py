@@ -104,3 +104,15 @@ def test_config_initial_states_are_intersection_free(cfg):
     o = OracleIPC(sc)
     assert o.check_collision() == 0
     assert sum(o.find_constraints()) == 0
+
+
+@pytest.mark.parametrize("make", [lambda: S.compressed_slab(6, 8, 6), lambda: S.two_cubes(3), S.tiny_pair])
+def test_dirichlet_rows_match_full_targets(make):
+    """The per-row prescribed positions equal the Dirichlet rows of the full
+    target array, at several times."""
+    sc = make()
+    mask = sc.dirichlet_mask
+    for t in (0.0, 0.013, 0.25):
+        rows, xyz = sc.dirichlet_rows(t)
+        assert rows.dtype == np.int32 and np.array_equal(rows, np.flatnonzero(mask))
+        assert np.array_equal(xyz, sc.dirichlet(t)[mask])
