@@ -113,7 +113,8 @@ def run_gpu(args, variant):
         if not os.path.exists(WEIGHTS):
             raise FileNotFoundError(f"{WEIGHTS} is missing (python -m fastipc_amd.warmstart train ...)")
         net = W.WarmStartMLP.load(WEIGHTS)
-        warm_meta = {k: net.meta.get(k) for k in ("family", "size", "seeds", "steps", "iters", "rows")}
+        warm_meta = {k: net.meta.get(k) for k in ("family", "size", "seeds", "steps", "iters", "rows", "onpolicy")}
+        warm_meta["rounds"] = len(net.meta.get("rounds", [])) or 1
         hook = W.LearnedWarmStart(net, W.scale_for(sc))
     elif ws == "extrapolation":
         # the zero network: the start repeats the previous step's correction

@@ -202,17 +202,21 @@ def family_scene(family: str, seed: int, size: str = "full"):
     raise ValueError(family)
 
 
-def capture_trajectories(mlp, family, seeds, steps, nsample=4096, size="full", log=print):
+def capture_trajectories(mlp, family, seeds, steps, nsample=4096, size="full", log=print, warm=None):
+    """Training rows from trajectories of the plain solver, or, with ``warm``
+    (a WarmStartMLP), of the solver warm-started by that model (on-policy
+    rows: the states the warm start itself produces)."""
     from .newton import NewtonIPC
     stats = []
     for sd in seeds:
         sc = family_scene(family, sd, size)
         sim = NewtonIPC(sc)
         cap = Capture(mlp, scale_for(sc), nsample, seed=sd)
+        hook = LearnedWarmStart(warm, scale_for(sc)) if warm is not None else None
         t0 = time.perf_counter()
         its = []
         for _ in range(steps):
-            st = sim.step(capture=cap)
+            st = sim.step(warm_start=hook, capture=cap)
             its.append(st.newton_iters)
         sim.close()
         stats.append(float(np.mean(its)))
@@ -301,9 +305,12 @@ def main(argv=None):
     tr.add_argument("--batch", type=int, default=8192)
     tr.add_argument("--lr", type=float, default=1e-3)
     tr.add_argument("--size", default="full")
-    tr.add_argument("--val-size", default="bench")
+    tr.add_argument("--val-size", default="full")
     tr.add_argument("--val-seed", type=int, default=1000)
     tr.add_argument("--out", required=True)
+    tr.add_argument("--init", default=None, help="continue from these weights")
+    tr.add_argument("--onpolicy", action="store_true",
+                    help="also capture trajectories warm-started by the initial model")
     ev = sub.add_parser("eval")
     ev.add_argument("--family", choices=("slab", "cubes"), default="slab")
     ev.add_argument("--weights", required=True)
@@ -322,9 +329,16 @@ def main(argv=None):
         print(m, flush=True)
 
     if a.cmd == "train":
-        mlp = WarmStartMLP(seed=12345)
+        mlp = WarmStartMLP.load(a.init) if a.init else WarmStartMLP(seed=12345)
         t0 = time.perf_counter()
         capture_trajectories(mlp, a.family, seeds(a.seeds), a.steps, a.nsample, size=a.size, log=log)
+        if a.onpolicy:
+            # the rows of the first pass stay in the dataset; the model that
+            # steps is a copy of the initial weights
+            behaviour = WarmStartMLP(params=mlp.params())
+            capture_trajectories(mlp, a.family, seeds(a.seeds), a.steps, a.nsample, size=a.size, log=log,
+                                 warm=behaviour)
+            behaviour.close()
         t1 = time.perf_counter()
         losses = []
         chunk = 250
@@ -339,7 +353,8 @@ def main(argv=None):
         meta = {"family": a.family, "seeds": a.seeds, "steps": a.steps, "nsample": a.nsample, "iters": a.iters,
                 "batch": a.batch, "lr": a.lr, "rows": int(len(X)), "final_loss": float(np.mean(losses[-100:])),
                 "zero_prediction_loss": base, "capture_s": t1 - t0, "train_s": t2 - t1,
-                "size": a.size, "validation_seed": a.val_seed, "validation_size": a.val_size, "validation": val}
+                "size": a.size, "validation_seed": a.val_seed, "validation_size": a.val_size, "validation": val,
+                "init": a.init, "onpolicy": bool(a.onpolicy)}
         mlp.save(a.out, meta)
         log(json.dumps(meta))
     else:
