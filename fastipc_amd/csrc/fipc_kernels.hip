@@ -162,9 +162,11 @@ __device__ __forceinline__ void tet_load(const int* tets, const double* IB, cons
 __host__ __device__ constexpr int uslot(int a, int c) { return a * 4 - a * (a - 1) / 2 + (c - a); }
 
 // compute_elasticity (NEWTON_IPC.py:568-691), first pass: one thread per tet
-// writes its ten upper 3x3 blocks to tetH[(slot * nt + e) * 9 + k] and its
-// gradient to tetg[(corner * nt + e) * 3 + i]; slot-major layout so that the
-// lanes of a wave store contiguous memory
+// writes each of its ten upper 3x3 blocks, oriented as the stored global
+// block, to tetH[cpos[10 e + slot] * 9 ..] (positions sorted by block, so the
+// gather reads contiguous runs) and its gradient to
+// tetg[(corner * nt + e) * 3 + i] (corner-major: a wave's stores are
+// contiguous)
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) k_elastic(int nt, const int* __restrict__ tets, const double* __restrict__ IB,
                                                  const double* __restrict__ vol, const double* __restrict__ la,
                                                  const double* __restrict__ mu, const double* __restrict__ x,
@@ -206,7 +208,7 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) k
 // second pass, one thread per stored (upper) block: sum its contiguous run of
 // oriented tet blocks in the scratch (tets in index order), add the lumped
 // mass on diagonal blocks (compute_inertia, NEWTON_IPC.py:558-567), store the
-// FP64 planes and, when valf != nullptr, their FP32 copy for the FP32 SpMV
+// FP64 values and, when valf != nullptr, their FP32 copy for the FP32 SpMV
 __global__ void k_gather_blocks(int nnzb, const int* __restrict__ bcptr, const double* __restrict__ tetH,
                                 const int* __restrict__ brow, const int* __restrict__ col,
                                 const double* __restrict__ m, double* __restrict__ val, float* __restrict__ valf)
@@ -541,15 +543,19 @@ __global__ void k_precond(int nv, const double* __restrict__ val, const int* __r
 // ---------------------------------------------------------------------------
 // SpMV: q += A p  (BSR part) and q += sum_rows H_r p_r (contact part)
 // ---------------------------------------------------------------------------
+// lanes per block row: with 4, a wavefront covers 16 consecutive rows, whose
+// gathers share cache lines of p and of the value stream (faster than 2, 8 or
+// 16 lanes on the 1M-tet slab)
 #define SPMV_G 4
 // One launch computes A p: blocks [0, nb_c) run the contact rows of every
 // type (first, so that they overlap the BSR rows instead of trailing them),
 // the nb_bsr blocks after them the BSR rows (nb_c is a multiple of 8, so the
 // XCD of a BSR block is that of its own index).
-//  - BSR rows: 8 lanes per block row stride over the row's entries (stored
-//    blocks, then transposed lower blocks) and reduce with shuffles; a block
-//    handles 32 consecutive rows per pass.  q[row] = (A_bsr p)[row] is
-//    stored (zero on Dirichlet rows), so q needs no clearing.
+//  - BSR rows: SPMV_G lanes per block row stride over the row's entries
+//    (stored blocks, then transposed lower blocks) and reduce with shuffles;
+//    a block handles TPB / SPMV_G consecutive rows per pass.  q[row] =
+//    (A_bsr p)[row] is stored (zero on Dirichlet rows), so q needs no
+//    clearing.
 //  - contact rows: a lane group per row, one lane per vertex; the lane's three
 //    rows of H_r p_r are added atomically into qc (skipping Dirichlet vertices).
 // With part != 0 every block stores its partial of p.(A p): p_row . q[row] for
