@@ -84,7 +84,7 @@ _SIGS = {
     "fipc_mlp_train": [_VP, C.c_int, C.c_int, C.c_float, _U64, _FP],
     "fipc_ws_features": [_VP, _D, _FP],
     "fipc_ws_predict": [_VP, _VP, _D],
-    "fipc_ws_capture_begin": [_VP, _VP, _D, C.c_int, _U64],
+    "fipc_ws_capture_begin": [_VP, _VP, _D, C.c_int, _U64, _D],
     "fipc_ws_capture_end": [_VP, _VP, _D],
 }
 _SCALAR = {

@@ -152,6 +152,7 @@ struct fipc_ctx {
     DVec<float> wsX;       // features, nv x 16
     DVec<unsigned long long> ckey;  // nearest contact row per vertex
     DVec<int> free_ids;    // vertices that are not Dirichlet (built on first capture)
+    DVec<int> near_ids, nnear;  // free vertices within d-hat of a contact primitive, their count
     int nfree = -1;
     DVec<int> cap_ids;     // vertices captured at the start of the step
     DVec<float> cap_X, cap_T;

@@ -1042,7 +1042,7 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     }
     release(c->ccd_pairs[0]); release(c->ccd_pairs[1]);
     release(c->dprev); release(c->wsX); release(c->ckey); release(c->free_ids); release(c->cap_ids);
-    release(c->cap_X); release(c->cap_T);
+    release(c->cap_X); release(c->cap_T); release(c->near_ids); release(c->nnear);
     (void)fipc_tm_fold(c);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->s);

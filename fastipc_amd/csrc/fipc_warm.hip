@@ -116,7 +116,31 @@ __global__ void k_ws_apply(int nv, const float* __restrict__ Y, int ldy, double 
     }
 }
 
-__global__ void k_ws_pick(int n, int nfree, const int* __restrict__ free_ids, unsigned long long seed,
+// free vertices within d-hat of a contact primitive (feature 9 = dist / d-hat
+// below 1), appended to near_ids; *nnear counts them
+__global__ void k_ws_near(int nfree, const int* __restrict__ free_ids, const float* __restrict__ feat, int* near_ids,
+                          int* nnear)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    bool near = false;
+    int v = 0;
+    if (k < nfree) {
+        v = free_ids[k];
+        near = feat[(size_t)v * WS_NF + 9] < 1.f;
+    }
+    // one counter update per wavefront
+    unsigned long long mask = __ballot(near);
+    if (!mask) return;
+    int lane = __lane_id(), leader = __ffsll((unsigned long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(nnear, __popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (near) near_ids[base + __popcll(mask & ((1ull << lane) - 1))] = v;
+}
+// sample k < nnear_samples from the near vertices (when there are any), the
+// others from all free vertices, uniformly with replacement
+__global__ void k_ws_pick(int n, int nfree, const int* __restrict__ free_ids, int nnear_samples,
+                          const int* __restrict__ near_ids, const int* __restrict__ nnear, unsigned long long seed,
                           const float* __restrict__ feat, int* ids, float* X)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -125,7 +149,9 @@ __global__ void k_ws_pick(int n, int nfree, const int* __restrict__ free_ids, un
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
     z ^= z >> 31;
-    int v = free_ids[z % (unsigned long long)nfree];
+    int nn = nnear_samples > 0 ? *nnear : 0;
+    int v = (k < nnear_samples && nn > 0) ? near_ids[z % (unsigned long long)nn]
+                                          : free_ids[z % (unsigned long long)nfree];
     ids[k] = v;
     for (int j = 0; j < WS_NF; j++) X[(size_t)k * WS_NF + j] = feat[(size_t)v * WS_NF + j];
 }
@@ -203,10 +229,11 @@ extern "C" int fipc_ws_predict(fipc_ctx* c, fipc_mlp* m, double scale)
     return FIPC_OK;
 }
 
-extern "C" int fipc_ws_capture_begin(fipc_ctx* c, fipc_mlp* m, double scale, int nsample, unsigned long long seed)
+extern "C" int fipc_ws_capture_begin(fipc_ctx* c, fipc_mlp* m, double scale, int nsample, unsigned long long seed,
+                                     double near_frac)
 {
     if (!c || !c->nv) return FIPC_ERR_STATE;
-    if (!m || nsample <= 0) return FIPC_ERR_ARG;
+    if (!m || nsample <= 0 || !(near_frac >= 0.0 && near_frac <= 1.0)) return FIPC_ERR_ARG;
     STCHK(ws_check_net(m));
     if (c->nfree < 0) {
         // free vertices, from the Dirichlet mask
@@ -228,8 +255,15 @@ extern "C" int fipc_ws_capture_begin(fipc_ctx* c, fipc_mlp* m, double scale, int
     STCHK(ensure(c->cap_ids, nsample));
     STCHK(ensure(c->cap_X, (size_t)nsample * WS_NF));
     STCHK(ensure(c->cap_T, (size_t)nsample * 3));
-    k_ws_pick<<<nblk(nsample), TPB, 0, c->s>>>(nsample, c->nfree, c->free_ids.p, seed, c->wsX.p, c->cap_ids.p,
-                                               c->cap_X.p);
+    int nnear_samples = (int)(near_frac * nsample);
+    if (nnear_samples > 0) {
+        STCHK(ensure(c->near_ids, c->nfree));
+        STCHK(ensure(c->nnear, 1));
+        HIPCHK(hipMemsetAsync(c->nnear.p, 0, 4, c->s));
+        k_ws_near<<<nblk(c->nfree), TPB, 0, c->s>>>(c->nfree, c->free_ids.p, c->wsX.p, c->near_ids.p, c->nnear.p);
+    }
+    k_ws_pick<<<nblk(nsample), TPB, 0, c->s>>>(nsample, c->nfree, c->free_ids.p, nnear_samples, c->near_ids.p,
+                                               c->nnear.p, seed, c->wsX.p, c->cap_ids.p, c->cap_X.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->s));
     c->cap_n = nsample;

@@ -161,15 +161,18 @@ class LearnedWarmStart:
 
 class Capture:
     """Hook for NewtonIPC.step(capture=...): records (features, target) rows of
-    nsample random free vertices per step into the network's training set."""
+    nsample random free vertices per step into the network's training set, a
+    share ``near_frac`` of them drawn among the vertices within d-hat of a
+    contact primitive."""
 
-    def __init__(self, mlp: WarmStartMLP, scale: float, nsample: int = 4096, seed: int = 0):
+    def __init__(self, mlp: WarmStartMLP, scale: float, nsample: int = 4096, seed: int = 0, near_frac: float = 0.0):
         self.mlp, self.scale, self.nsample, self.seed = mlp, scale, nsample, seed
+        self.near_frac = near_frac
         self.k = 0
 
     def begin(self, sim):
         _lib.call("fipc_ws_capture_begin", sim.ctx, self.mlp.h, self.scale, self.nsample,
-                  self.seed * 1000003 + self.k)
+                  self.seed * 1000003 + self.k, float(self.near_frac))
         self.k += 1
 
     def end(self, sim):
@@ -202,7 +205,7 @@ def family_scene(family: str, seed: int, size: str = "full"):
     raise ValueError(family)
 
 
-def capture_trajectories(mlp, family, seeds, steps, nsample=4096, size="full", log=print, warm=None):
+def capture_trajectories(mlp, family, seeds, steps, nsample=4096, size="full", log=print, warm=None, near_frac=0.0):
     """Training rows from trajectories of the plain solver, or, with ``warm``
     (a WarmStartMLP), of the solver warm-started by that model (on-policy
     rows: the states the warm start itself produces)."""
@@ -211,7 +214,7 @@ def capture_trajectories(mlp, family, seeds, steps, nsample=4096, size="full", l
     for sd in seeds:
         sc = family_scene(family, sd, size)
         sim = NewtonIPC(sc)
-        cap = Capture(mlp, scale_for(sc), nsample, seed=sd)
+        cap = Capture(mlp, scale_for(sc), nsample, seed=sd, near_frac=near_frac)
         hook = LearnedWarmStart(warm, scale_for(sc)) if warm is not None else None
         t0 = time.perf_counter()
         its = []
@@ -311,6 +314,8 @@ def main(argv=None):
     tr.add_argument("--init", default=None, help="continue from these weights")
     tr.add_argument("--onpolicy", action="store_true",
                     help="also capture trajectories warm-started by the initial model")
+    tr.add_argument("--near-frac", type=float, default=0.0,
+                    help="share of each step's samples drawn near contacts")
     ev = sub.add_parser("eval")
     ev.add_argument("--family", choices=("slab", "cubes"), default="slab")
     ev.add_argument("--weights", required=True)
@@ -331,13 +336,14 @@ def main(argv=None):
     if a.cmd == "train":
         mlp = WarmStartMLP.load(a.init) if a.init else WarmStartMLP(seed=12345)
         t0 = time.perf_counter()
-        capture_trajectories(mlp, a.family, seeds(a.seeds), a.steps, a.nsample, size=a.size, log=log)
+        capture_trajectories(mlp, a.family, seeds(a.seeds), a.steps, a.nsample, size=a.size, log=log,
+                             near_frac=a.near_frac)
         if a.onpolicy:
             # the rows of the first pass stay in the dataset; the model that
             # steps is a copy of the initial weights
             behaviour = WarmStartMLP(params=mlp.params())
             capture_trajectories(mlp, a.family, seeds(a.seeds), a.steps, a.nsample, size=a.size, log=log,
-                                 warm=behaviour)
+                                 warm=behaviour, near_frac=a.near_frac)
             behaviour.close()
         t1 = time.perf_counter()
         losses = []
@@ -354,7 +360,7 @@ def main(argv=None):
                 "batch": a.batch, "lr": a.lr, "rows": int(len(X)), "final_loss": float(np.mean(losses[-100:])),
                 "zero_prediction_loss": base, "capture_s": t1 - t0, "train_s": t2 - t1,
                 "size": a.size, "validation_seed": a.val_seed, "validation_size": a.val_size, "validation": val,
-                "init": a.init, "onpolicy": bool(a.onpolicy)}
+                "init": a.init, "onpolicy": bool(a.onpolicy), "near_frac": a.near_frac}
         mlp.save(a.out, meta)
         log(json.dumps(meta))
     else:

@@ -260,12 +260,15 @@ int fipc_mlp_train(fipc_mlp* m, int iters, int batch, float lr, unsigned long lo
  * their prescribed increment.  p is the solution vector, to be filtered by
  * fipc_ccd_alpha and applied with fipc_apply_sol.
  * fipc_ws_capture_begin keeps the features of nsample free vertices drawn
- * from seed; fipc_ws_capture_end (after the step, before fipc_end_step)
+ * from seed, a share near_frac of them (when there are any) among the
+ * vertices within d-hat of a contact primitive (feature 9 below 1);
+ * fipc_ws_capture_end (after the step, before fipc_end_step)
  * appends them to the training set with targets
  * ((x_{n+1} - xTilde) - c_prev) / scale. */
 int fipc_ws_features(fipc_ctx* c, double scale, float* feat);
 int fipc_ws_predict(fipc_ctx* c, fipc_mlp* m, double scale);
-int fipc_ws_capture_begin(fipc_ctx* c, fipc_mlp* m, double scale, int nsample, unsigned long long seed);
+int fipc_ws_capture_begin(fipc_ctx* c, fipc_mlp* m, double scale, int nsample, unsigned long long seed,
+                          double near_frac);
 int fipc_ws_capture_end(fipc_ctx* c, fipc_mlp* m, double scale);
 
 #ifdef __cplusplus

@@ -232,3 +232,26 @@ def test_warm_started_steps_match_plain(scene_fn, steps, amp):
         assert err <= 1e-6 * sc.bbox_diag, (k, err, sa.newton_iters, sb.newton_iters)
     print(f"warm start accepted in {accepted} of {steps} steps")
     m.close()
+
+
+def test_capture_near_contacts():
+    """With near_frac = 1 every captured row comes from a vertex within d-hat
+    of a contact primitive (feature 9 below 1), when the step has any."""
+    sc = S.two_cubes(4)
+    sim = NewtonIPC(sc)
+    for _ in range(3):
+        sim.step()
+    m = WarmStartMLP(seed=9)
+    sim.compute_xn_and_xTilde(sim.frame * sc.dt)
+    sim.find_constraints()
+    f = np.zeros((sim.nv, 16), np.float32)
+    _lib.call("fipc_ws_features", sim.ctx, scale_for(sc), _lib.fptr(f))
+    near = (f[:, 9] < 1) & ~sc.dirichlet_mask
+    cap = Capture(m, scale_for(sc), nsample=256, seed=4, near_frac=1.0)
+    sim.step(capture=cap)
+    X, _ = m.dataset()
+    assert len(X) == 256
+    if near.any():
+        assert np.all(X[:, 9] < 1), (near.sum(), X[:, 9].max())
+    print(f"{near.sum()} vertices near contacts")
+    m.close()
