@@ -186,6 +186,21 @@ def run_gpu(args, variant):
             "avg_launch_us": round(kt[dom]["us_avg"], 2), "launches": kt[dom]["launches"]}
     others = {k: {"achieved_GBs": round(nbytes[k] / (kt[k]["us_avg"] * 1e-6) / 1e9, 1) if kt[k]["us_avg"] else None,
                   "avg_launch_us": round(kt[k]["us_avg"], 2)} for k in nbytes}
+    # CCD pass (point-triangle and edge-edge candidate queries on the swept
+    # boxes, then the narrow phases: four timed launches per pass); its
+    # compulsory unique bytes are x and the direction at the surface vertices,
+    # the primitive indices and 16 bytes per candidate pair of the last pass
+    pairs = np.zeros(2, np.int32)
+    _lib.call("fipc_ccd_counts", sim.ctx, _lib.iptr(pairs))
+    s_ = sim.setup
+    nbp, nbe, nbt = len(s_.bp), len(s_.be), len(s_.bt)
+    ccd_bytes = nbp * 48 + nbp * 4 + nbe * 8 + nbt * 12 + int(pairs.sum()) * 16
+    passes = kt["ccd_query"]["launches"] // 4
+    if passes:
+        pass_us = kt["ccd_query"]["ms_total"] / passes * 1e3
+        others["ccd"] = {"achieved_GBs": round(ccd_bytes / (pass_us * 1e-6) / 1e9, 1), "avg_pass_us": round(pass_us, 2),
+                         "unique_bytes_per_pass": ccd_bytes, "candidate_pairs_last_pass": [int(p) for p in pairs],
+                         "bound": "FP64 VALU / latency"}
     res = {
         "variant": variant,
         "warm_start_kept": accepted,

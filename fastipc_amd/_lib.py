@@ -53,6 +53,7 @@ _SIGS = {
     "fipc_residual": [_VP, _P],
     "fipc_energy": [_VP, _P, _P],
     "fipc_ccd_alpha": [_VP, _D, C.c_int, _P, _P],
+    "fipc_ccd_counts": [_VP, _IP],
     "fipc_save_prev": [_VP],
     "fipc_apply_sol": [_VP, _D],
     "fipc_check_collision": [_VP, _IP],

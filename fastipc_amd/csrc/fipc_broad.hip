@@ -695,10 +695,20 @@ extern "C" int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* al
         }
         if (retry) continue;
         memcpy(a, out, 24);
+        c->ccd_last[0] = hc[0];
+        c->ccd_last[1] = hc[1];
         break;
     }
     if (parts) { parts[0] = a[0]; parts[1] = a[1]; parts[2] = a[2]; }
     *alpha = std::min(std::min(a[0], a[1]), a[2]);
+    return FIPC_OK;
+}
+
+extern "C" int fipc_ccd_counts(fipc_ctx* c, int* pairs)
+{
+    if (!c || !pairs) return FIPC_ERR_ARG;
+    pairs[0] = c->ccd_last[0];
+    pairs[1] = c->ccd_last[1];
     return FIPC_OK;
 }
 

@@ -144,6 +144,7 @@ struct fipc_ctx {
     // CCD candidate pairs (point-triangle, edge-edge) between the grid walk and the narrow phase
     int ccd_cap = 0;
     DVec<int2> ccd_pairs[2];
+    int ccd_last[2] = {0, 0};  // candidate pairs (PT, EE) of the last fipc_ccd_alpha
     // broad phase grids
     Grid grid[2];                         // [0] boundary points, [1] boundary edges
     unsigned long long x_version = 1;     // incremented by every write of x

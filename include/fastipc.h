@@ -178,6 +178,9 @@ int fipc_energy(fipc_ctx* c, double* E, double* parts);
 /* compute_intersection_free_step_size (NEWTON_IPC.py:456-468): PT and EE CCD
  * filters and (inversion != 0) the per-tet inversion filter; parts[3] may be NULL */
 int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* alpha, double* parts);
+/* candidate pairs (point-triangle, edge-edge) that passed the lower-bound
+ * filter in the last fipc_ccd_alpha and went to the narrow phase */
+int fipc_ccd_counts(fipc_ctx* c, int* pairs);
 /* save_xPrev (NEWTON_IPC.py:786-789) */
 int fipc_save_prev(fipc_ctx* c);
 /* apply_sol (NEWTON_IPC.py:792-796): x = xPrev + alpha p */
