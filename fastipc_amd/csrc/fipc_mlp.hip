@@ -54,7 +54,9 @@ struct GemmArgs {
     float* Zout;         // EPI_BIAS_SILU: pre-activation output (may be null)
 };
 
-__device__ __forceinline__ float sigm(float z) { return 1.f / (1.f + __expf(-z)); }
+// logistic sigmoid with the hardware reciprocal (about 1 ulp): the products
+// that consume it round to BF16
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
 
 __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g)
 {
