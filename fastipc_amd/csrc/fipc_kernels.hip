@@ -78,6 +78,22 @@ __device__ __forceinline__ double reduce_partials(const double* part, int n)
     __syncthreads();
     return r;
 }
+// two partial arrays reduced in one pass (their loads overlap and the block
+// synchronises once); every thread receives both sums, identical in every
+// block.  Called at most once per kernel.
+__device__ __forceinline__ void reduce_partials2(const double* a, const double* b, int n, double& ra, double& rb)
+{
+    __shared__ double sh[2][TPB / 64];
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) { s0 += a[i]; s1 += b[i]; }
+    s0 = warp_sum(s0);
+    s1 = warp_sum(s1);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sh[0][w] = s0; sh[1][w] = s1; }
+    __syncthreads();
+    ra = 0.0; rb = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) { ra += sh[0][i]; rb += sh[1][i]; }
+}
 
 // Cross-block reductions never synchronise blocks inside a kernel.  On gfx950
 // the L2 is private to each XCD, so an agent-scope release / acquire pair
@@ -785,7 +801,7 @@ template <class V, class M>
 __global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const unsigned char* __restrict__ dmask,
                                                   const M* __restrict__ Minv, const V* __restrict__ p,
                                                   const V* __restrict__ q, double* __restrict__ qc,
-                                                  double* x, V* r, V* z,
+                                                  double* __restrict__ x, V* __restrict__ r, V* __restrict__ z,
                                                   const double* part_pq, int npq, double* part_rz, double* part_rr,
                                                   const int* state, int it)
 {
@@ -796,21 +812,34 @@ __global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const un
     double rz = 0.0, rr = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
         if (dmask[i]) continue;
-        double ri[3];
+        // every load of the vertex is issued before the first store
+        const int k = 3 * i;
+        double pv[3], qv[3], qcv[3], xv[3], rv[3], Mi[9];
+#pragma unroll
         for (int d = 0; d < 3; d++) {
-            x[3 * i + d] += alpha * (double)p[3 * i + d];
+            pv[d] = (double)p[k + d];
+            qv[d] = (double)q[k + d];
+            qcv[d] = qc[k + d];
+            xv[d] = x[k + d];
+            rv[d] = (double)r[k + d];
+        }
+#pragma unroll
+        for (int j = 0; j < 9; j++) Mi[j] = (double)Minv[9 * i + j];
+        double ri[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            x[k + d] = xv[d] + alpha * pv[d];
             // qc is non-zero only at contact vertices: cleared here, where
             // it is consumed, for the next product
-            double qcv = qc[3 * i + d];
-            if (qcv != 0.0) qc[3 * i + d] = 0.0;
-            V rv = (V)((double)r[3 * i + d] - alpha * ((double)q[3 * i + d] + qcv));
-            r[3 * i + d] = rv;
-            ri[d] = (double)rv;
+            if (qcv[d] != 0.0) qc[k + d] = 0.0;
+            V rn = (V)(rv[d] - alpha * (qv[d] + qcv[d]));
+            r[k + d] = rn;
+            ri[d] = (double)rn;
         }
-        const M* Mi = Minv + 9 * i;
+#pragma unroll
         for (int d = 0; d < 3; d++) {
-            V zv = (V)((double)Mi[3 * d] * ri[0] + (double)Mi[3 * d + 1] * ri[1] + (double)Mi[3 * d + 2] * ri[2]);
-            z[3 * i + d] = zv;
+            V zv = (V)(Mi[3 * d] * ri[0] + Mi[3 * d + 1] * ri[1] + Mi[3 * d + 2] * ri[2]);
+            z[k + d] = zv;
             rz += ri[d] * (double)zv;
             rr += ri[d] * ri[d];
         }
@@ -825,13 +854,13 @@ __global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const un
 // reduces r.z and r.r, runs the convergence test (block 0 records it),
 // beta = rz_new / rz; p = z + beta p
 template <class V>
-__global__ void __launch_bounds__(TPB) k_pcg_upd2(int nv, double* scal, const V* __restrict__ z, V* p,
+__global__ void __launch_bounds__(TPB) k_pcg_upd2(int nv, double* scal, const V* __restrict__ z, V* __restrict__ p,
                                                   const double* part_rz, const double* part_rr, int n1,
                                                   int* state, int it, double tol2)
 {
     if (it > state[0]) return;
-    double rz_new = reduce_partials(part_rz, n1);
-    double rr = reduce_partials(part_rr, n1);
+    double rz_new, rr;
+    reduce_partials2(part_rz, part_rr, n1, rz_new, rr);
     double beta = rz_new / scal[it & 1];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         scal[(it + 1) & 1] = rz_new;
