@@ -1,13 +1,17 @@
 """Benchmark: simulation steps/s of the IPC Newton step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--mode chosen|plain|fp32|fp32+extrapolation] [--all-variants]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode VARIANT] [--all-variants]
+                    [--pcg-tol 1e-6] [--pcg-tol-search 1e-2]
 
-The value is the steps/s of --mode (default "chosen": the learned Newton warm
-start with the FP32 Hessian and FP64 residual refinement); the plain FP64
-method (x = x_n start, FP64 Hessian) is timed in the same run for the ratio.
-Every variant uses the same Newton tolerance and checks every step for
-intersections.
+The value is the steps/s of --mode (default "chosen+inexact": the learned
+Newton warm start with the FP32 Hessian and FP64 residual refinement, search
+directions solved to --pcg-tol-search).  The plain FP64 method (x = x_n start,
+FP64 Hessian, every direction solved to --pcg-tol: the reference's method with
+PCG for the direct solve) is timed in the same run for the ratio, and so is
+the plain method with the inexact search directions.  Every variant uses the
+same Newton test, on a direction solved to --pcg-tol (the inexact variants
+continue the solve of the direction that passes the test to --pcg-tol and test
+again before they accept), and checks every step for intersections.
 
 Workload: the compressed slab, a 50 x 80 x 50-vertex lattice (200,000
 vertices, 948,395 tets, E = 1e6, nu = 0.45) between two Dirichlet plates that
@@ -87,12 +91,17 @@ def algorithmic_bytes(sim, precision):
     }
 
 
-# the variants of the step: (FP32 Hessian, warm start: None | "learned" | "extrapolation")
+# the variants of the step: (FP32 Hessian, warm start: None | "learned" |
+# "extrapolation", inexact search directions)
 VARIANTS = {
-    "plain": (False, None),               # the reference's method on the GPU: FP64, x = x_n start
-    "fp32": (True, None),
-    "fp32+extrapolation": (True, "extrapolation"),
-    "chosen": (True, "learned"),          # learned warm start + FP32 Hessian
+    "plain": (False, None, False),        # the reference's method on the GPU: FP64, x = x_n start
+    "fp32": (True, None, False),
+    "fp32+extrapolation": (True, "extrapolation", False),
+    "chosen": (True, "learned", False),   # learned warm start + FP32 Hessian
+    # search directions solved to --pcg-tol-search, the direction that passes
+    # the Newton test continued to --pcg-tol before the test is repeated
+    "plain+inexact": (False, None, True),
+    "chosen+inexact": (True, "learned", True),
 }
 WEIGHTS = os.path.join(ROOT, "fastipc_amd", "weights", "ws_slab.npz")
 
@@ -104,10 +113,11 @@ def run_gpu(args, variant):
     from fastipc_amd import warmstart as W
 
     sc = S.compressed_slab(50, 80, 50, steps=args.warmup + args.steps + args.roofline_steps)
-    fp32, ws = VARIANTS[variant]
+    fp32, ws, inexact = VARIANTS[variant]
     precision = 1 if fp32 else 0
     t0 = time.perf_counter()
-    sim = NewtonIPC(sc, precision=precision, pcg_tol=args.pcg_tol)
+    sim = NewtonIPC(sc, precision=precision, pcg_tol=args.pcg_tol,
+                    pcg_tol_search=args.pcg_tol_search if inexact else None)
     hook, net, warm_meta = None, None, None
     if ws == "learned":
         if not os.path.exists(WEIGHTS):
@@ -260,11 +270,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=14)
-    ap.add_argument("--mode", choices=tuple(VARIANTS), default="chosen",
-                    help="the variant whose steps/s is the value (default: learned warm start + FP32)")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run timed for the ratio")
-    ap.add_argument("--all-variants", action="store_true", help="also time fp32 and fp32+extrapolation")
+    ap.add_argument("--mode", choices=tuple(VARIANTS), default="chosen+inexact",
+                    help="the variant whose steps/s is the value (default: learned warm start + FP32 Hessian "
+                         "+ inexact search directions)")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain FP64 runs (exact and inexact directions) timed for the ratios")
+    ap.add_argument("--all-variants", action="store_true", help="also time every other variant")
     ap.add_argument("--pcg-tol", type=float, default=1e-6)
+    ap.add_argument("--pcg-tol-search", type=float, default=1e-2,
+                    help="PCG tolerance of the search directions of the +inexact variants")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-warmup", type=int, default=6)
@@ -283,7 +297,7 @@ def main():
     if g is None:
         return
     others = {}
-    extra = [] if args.no_plain or args.mode == "plain" else ["plain"]
+    extra = [] if args.no_plain else [v for v in ("plain", "plain+inexact") if v != args.mode]
     if args.all_variants:
         extra += [v for v in VARIANTS if v not in (args.mode, "plain")]
     for v in extra:
@@ -293,7 +307,7 @@ def main():
                      "pcg_iters_per_newton": round(r["pcg_iters_per_newton"], 1),
                      "warm_start_kept": r["warm_start_kept"]}
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(args)
-    fp32, ws = VARIANTS[args.mode]
+    fp32, ws, inexact = VARIANTS[args.mode]
     line = {
         "metric": "sim steps/s at 1M tets, warm-start+FP32 vs plain FP64 GPU at equal residual",
         "value": round(g["steps_per_s"], 4),
@@ -314,9 +328,15 @@ def main():
                    "pcg_iters_per_newton": round(g["pcg_iters_per_newton"], 1),
                    "warm_start_kept": g["warm_start_kept"],
                    "newton_tol": "1e-2 dt bbox on max|p| (every variant)", "intersections": 0,
-                   "pcg_rel_tol": args.pcg_tol, "contacts_last_step": g["contacts_last_step"]},
+                   "pcg_rel_tol": args.pcg_tol,
+                   "pcg_rel_tol_search_directions": args.pcg_tol_search if inexact else args.pcg_tol,
+                   "contacts_last_step": g["contacts_last_step"]},
         "same_run_variants": others,
         "speedup_vs_plain_fp64": round(g["steps_per_s"] / others["plain"]["steps_per_s"], 4) if "plain" in others else None,
+        # the same ratio against the plain method given the inexact search
+        # directions too: what the warm start and the FP32 Hessian add
+        "speedup_vs_plain_fp64_inexact": (round(g["steps_per_s"] / others["plain+inexact"]["steps_per_s"], 4)
+                                          if "plain+inexact" in others else None),
         "roofline": g["roofline"],
         "cpu_baseline": cpu,
         "kernel_classes": g["kernel_classes"],

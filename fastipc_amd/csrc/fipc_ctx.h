@@ -162,6 +162,10 @@ struct fipc_ctx {
     double last_pcg_relres = 0.0;
     int last_pcg_iters = 0;
     int pcg_hint = 0;      // iterations of the last first round, sizes the first chunk
+    int refine_hint = 0;   // iterations of the first round of the last continuation
+    // fipc_assemble bumps sys_version; fipc_solve records the version it
+    // solved, which fipc_solve_refine requires
+    unsigned long long sys_version = 1, solve_version = 0;
     KernelTimer tm[FIPC_NTIMERS];
     std::vector<hipEvent_t> event_pool;
     bool timing = false;

@@ -1192,6 +1192,7 @@ extern "C" int fipc_upload_mesh(fipc_ctx* c, int nv, const double* x0, const dou
     STCHK(ensure(c->cdiag, (size_t)nv * 9)); STCHK(ensure(c->Minv, (size_t)nv * 9));
     HIPCHK(hipMemcpyAsync(c->x.p, x0, n3 * 8, hipMemcpyHostToDevice, c->s));
     c->x_version++;
+    c->sys_version++;
     HIPCHK(hipMemcpyAsync(c->xt.p, x0, n3 * 8, hipMemcpyHostToDevice, c->s));
     HIPCHK(hipMemsetAsync(c->v.p, 0, n3 * 8, c->s));
     STCHK(ensure(c->dprev, n3));
@@ -1280,6 +1281,7 @@ extern "C" int fipc_set_vector(fipc_ctx* c, int which, const double* in)
     if (!d || !in) return FIPC_ERR_ARG;
     HIPCHK(hipMemcpyAsync(d->p, in, (size_t)c->nv * 24, hipMemcpyHostToDevice, c->s));
     if (d == &c->x) c->x_version++;
+    if (d != &c->sol) c->sys_version++;   // a continued solve needs the system it solved
     HIPCHK(hipStreamSynchronize(c->s));
     return FIPC_OK;
 }
@@ -1375,6 +1377,7 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
 {
     if (!c || !c->nv) return FIPC_ERR_STATE;
     c->precision = precision;
+    c->sys_version++;
     STCHK(fipc_tm_begin(c, TM_ELASTIC));
     if (c->nt)
         k_elastic<<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p, c->x.p,
@@ -1466,7 +1469,7 @@ extern "C" int fipc_spmv(fipc_ctx* c, const double* x, double* y)
 // Hessian; FP32 in the FP32 Hessian's refinement rounds)
 template <class V, class M>
 static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* q, int round, int budget,
-                     double tol2, int state[2])
+                     double tol2, int hint, int state[2])
 {
     const int nv = c->nv, nbv = nblk(nv), nsp = spmv_nparts(c), nu = pcg_grid(nv), chunk = 8;
     // r = b - A x with the FP64 blocks (q + qc), z = Minv r, p = z
@@ -1486,8 +1489,9 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
         if (state[0] == -1) return FIPC_OK;
     }
     // the host checks convergence between chunks of iterations; the first
-    // chunk is sized from the previous solve, whose count is usually close
-    int first = round == 0 ? std::max(chunk, c->pcg_hint - chunk / 2) : chunk;
+    // chunk is sized from a previous solve of the same kind (hint), whose
+    // count is usually close
+    int first = std::max(chunk, hint - chunk / 2);
     while (it < budget) {
         int end = std::min(budget, it + (it == 0 ? first : chunk));
         for (; it < end; it++) {
@@ -1511,16 +1515,16 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
     return FIPC_OK;
 }
 
-// solve_system (NEWTON_IPC.py:750-777): block-Jacobi PCG from the prescribed
-// Dirichlet increments.  With the FP32 Hessian the iterations use the FP32
-// copy of the elastic blocks, of the preconditioner and of the iteration
-// vectors, and are restarted from the residual b - A x recomputed with the
-// FP64 blocks until that residual meets the tolerance (iterative
-// refinement; the solution accumulates in FP64); with the FP64 Hessian there
-// is a single FP64 round.
-extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double* relres)
+// PCG rounds from first_round on.  Round 0 starts from the prescribed
+// Dirichlet increments; a round > 0 restarts from the current solution with
+// the residual b - A x recomputed with the FP64 blocks, against the reference
+// norm |r0| of the last round 0.  With the FP32 Hessian the iterations use the
+// FP32 copy of the elastic blocks, of the preconditioner and of the iteration
+// vectors, and rounds repeat until the recomputed FP64 residual meets the
+// tolerance (iterative refinement; the solution accumulates in FP64); with the
+// FP64 Hessian a call runs a single round.
+static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int* iters, double* relres)
 {
-    if (!c || !c->nv) return FIPC_ERR_STATE;
     int nv = c->nv;
     int nbv = nblk(nv);
     const int nsp = spmv_nparts(c);
@@ -1534,7 +1538,7 @@ extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double
     const int rounds = prec == 1 ? 4 : 1;
     int total = 0, state[2] = {0, 0};
     bool converged = false;
-    for (int round = 0; round < rounds; round++) {
+    for (int round = first_round; round < first_round + rounds; round++) {
         if (round == 0) {
             // x0 = prescribed Dirichlet increments
             k_pcg_x0<<<nbv, TPB, 0, c->s>>>(nv, c->xt.p, c->x.p, c->dmask.p, c->sol.p, c->qc.p);
@@ -1543,16 +1547,20 @@ extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double
             HIPCHK(hipMemsetAsync(c->qc.p, 0, (size_t)nv * 24, c->s));
         }
         int budget = maxit - total;
+        // first chunk: the last first round's count, or that of the last
+        // continuation (fipc_solve_refine) for its first round
+        int hint = round == 0 ? c->pcg_hint : (round == first_round ? c->refine_hint : 0);
         if (prec == 1)
             STCHK((pcg_round<float, float>(c, prec, c->Minvf.p, c->rf.p, c->zf.p, c->pf.p, c->qf.p, round, budget,
-                                           tol2, state)));
+                                           tol2, hint, state)));
         else
             STCHK((pcg_round<double, double>(c, prec, c->Minv.p, c->r.p, c->z.p, c->pp.p, c->q.p, round, budget,
-                                             tol2, state)));
+                                             tol2, hint, state)));
         if (state[0] == -1 && round > 0) { converged = true; break; }
         int ran = state[0] == -1 ? 0 : state[1];
         total += ran;
         if (round == 0 && ran > 0) c->pcg_hint = ran;
+        if (round == first_round && first_round > 0 && ran > 0) c->refine_hint = ran;
         if (state[0] == 0x7fffffff) break;        // iteration budget spent
         if (prec != 1 || state[0] == -1) { converged = true; break; }
         // FP32 iterations converged: the next round checks the FP64 residual
@@ -1571,6 +1579,22 @@ extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double
     if (!(rel == rel)) return FIPC_ERR_NONFINITE;
     if (!converged) return FIPC_ERR_NOT_CONVERGED;
     return FIPC_OK;
+}
+// solve_system (NEWTON_IPC.py:750-777): block-Jacobi PCG from the prescribed
+// Dirichlet increments
+extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double* relres)
+{
+    if (!c || !c->nv) return FIPC_ERR_STATE;
+    int rc = solve_rounds(c, 0, tol, maxit, iters, relres);
+    c->solve_version = c->sys_version;
+    return rc;
+}
+// continues the last fipc_solve of the current system from its solution to
+// the tighter tolerance tol (relative to the same |r0|)
+extern "C" int fipc_solve_refine(fipc_ctx* c, double tol, int maxit, int* iters, double* relres)
+{
+    if (!c || !c->nv || c->solve_version != c->sys_version) return FIPC_ERR_STATE;
+    return solve_rounds(c, 1, tol, maxit, iters, relres);
 }
 
 // ---- energy ----

@@ -17,6 +17,13 @@ of CHOLMOD; Dirichlet vertices whose prescribed position moves between steps
 reach it through a prescribed increment in the solve (static Dirichlet
 vertices behave exactly as in the reference); safety caps bound the Newton and
 halving loops, which the reference leaves unbounded.
+
+Inexact search directions (``pcg_tol_search``, off by default): the solves
+whose direction is applied stop at the looser relative tolerance
+``pcg_tol_search``; when such a direction passes the convergence test, the
+solve is continued to ``pcg_tol`` (``fipc_solve_refine``) and the test is
+repeated on the refined direction, so a step is accepted only on a direction
+solved to ``pcg_tol``, as without the option.
 """
 from __future__ import annotations
 
@@ -48,13 +55,18 @@ class StepStats:
 
 class NewtonIPC:
     def __init__(self, scene, precision: int = 0, pcg_tol: float = 1e-6, pcg_maxit: int = 20000,
-                 max_newton: int = 200, max_halvings: int = 60, inversion: bool = True, timing: bool = False):
+                 max_newton: int = 200, max_halvings: int = 60, inversion: bool = True, timing: bool = False,
+                 pcg_tol_search: float | None = None):
         self.so = _lib.load()
         self.scene = scene
         self.setup = s = SceneSetup(scene)
         self.precision = precision
         self.pcg_tol = pcg_tol
+        if pcg_tol_search is not None and not pcg_tol_search >= pcg_tol:
+            raise ValueError("pcg_tol_search must be at least pcg_tol")
+        self.pcg_tol_search = pcg_tol_search
         self.pcg_maxit = pcg_maxit
+        self.last_pcg_iters = 0
         self.max_newton = max_newton
         self.max_halvings = max_halvings
         self.inversion = inversion
@@ -172,12 +184,26 @@ class NewtonIPC:
     def compute_hessian_and_gradient(self):
         _lib.call("fipc_assemble", self.ctx, self.precision)
 
-    def solve_system(self, t: float = 0.0):
+    def solve_system(self, t: float = 0.0, tol: float | None = None):
         it = C.c_int()
         rel = C.c_double()
-        code = self.so.fipc_solve(self.ctx, self.pcg_tol, self.pcg_maxit, C.byref(it), C.byref(rel))
+        tol = self.pcg_tol if tol is None else tol
+        code = self.so.fipc_solve(self.ctx, tol, self.pcg_maxit, C.byref(it), C.byref(rel))
         if code not in (0, 5):
             raise _lib.FipcError("fipc_solve", code)
+        self.last_pcg_iters = it.value
+        return it.value, rel.value
+
+    def refine_solve(self, tol: float | None = None):
+        """Continues the last solve of the current system to ``tol``
+        (default ``pcg_tol``) from its solution."""
+        it = C.c_int()
+        rel = C.c_double()
+        tol = self.pcg_tol if tol is None else tol
+        maxit = max(self.pcg_maxit - self.last_pcg_iters, 1)
+        code = self.so.fipc_solve_refine(self.ctx, tol, maxit, C.byref(it), C.byref(rel))
+        if code not in (0, 5):
+            raise _lib.FipcError("fipc_solve_refine", code)
         return it.value, rel.value
 
     def output_residual(self) -> float:
@@ -243,9 +269,15 @@ class NewtonIPC:
                     with T("assemble"):
                         self.compute_hessian_and_gradient()
                 with T("solve"):
-                    it, rel = self.solve_system(f * dt)
-                    st.pcg_iters.append(it)
+                    it, rel = self.solve_system(f * dt, self.pcg_tol_search)
                 res = self.output_residual()
+                if self.pcg_tol_search is not None and res < self.setup.newton_tol:
+                    # the direction that decides convergence is solved to pcg_tol
+                    with T("solve"):
+                        it2, rel = self.refine_solve()
+                    it += it2
+                    res = self.output_residual()
+                st.pcg_iters.append(it)
                 st.residuals.append(res)
                 if res < self.setup.newton_tol:
                     break

@@ -171,6 +171,14 @@ int fipc_assemble(fipc_ctx* c, int precision);
 /* solve_system (NEWTON_IPC.py:750-777) with block-Jacobi PCG instead of
  * CHOLMOD: relative residual tolerance tol, at most maxit iterations */
 int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double* relres);
+/* continues the last fipc_solve of the current system (no fipc_assemble,
+ * fipc_upload_mesh or fipc_set_vector of anything but the solution since)
+ * from its solution to the relative tolerance tol, against the same initial
+ * residual norm; iters counts the added iterations.  Lets a caller run the
+ * search directions of an inexact Newton method at a loose tolerance and
+ * tighten only the solve whose direction decides convergence.  No
+ * counterpart in the reference, whose direct solve is exact. */
+int fipc_solve_refine(fipc_ctx* c, double tol, int maxit, int* iters, double* relres);
 /* output_residual (NEWTON_IPC.py:805-812): max |p_i| */
 int fipc_residual(fipc_ctx* c, double* res);
 /* compute_energy (NEWTON_IPC.py:514-542); parts[3] = inertia, elastic, barrier (may be NULL) */

@@ -5,6 +5,8 @@ tolerance.  Checked here: the refined direction meets the relative tolerance
 measured with the FP64 Hessian, it agrees with the FP64-mode direction, and
 whole time steps agree with the oracle within 1e-6 of the bbox diagonal at a
 tightened Newton tolerance."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -103,4 +105,69 @@ def test_fp32_steps_match_oracle(scene_fn, steps, amp):
         assert g.check_collision() == 0
         err = np.abs(g.x - o.x).max()
         print(f"step {f}: newton {sg.newton_iters} vs {so['newton_iters']}, position error {err / sc.bbox_diag:.3e} bbox")
+        assert err <= 1e-6 * sc.bbox_diag, (f, err)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_solve_refine_continues_to_tight_tolerance(prec):
+    """fipc_solve at a loose tolerance, then fipc_solve_refine to a tight one:
+    the continued solution meets the tight tolerance in the FP64 residual
+    (relative to the same |b|), agrees with a direct tight solve, and the
+    continuation is refused once the system has been reassembled."""
+    sc = S.two_cubes(5)
+    x = _deformed(sc, 0.004, 5)
+    loose, tight = 1e-2, 1e-9
+    g = NewtonIPC(sc, precision=prec, pcg_tol=tight, pcg_tol_search=loose)
+    ref = NewtonIPC(sc, precision=prec, pcg_tol=tight)
+    for s in (g, ref):
+        s.set_state(x)
+        s.compute_xn_and_xTilde(0.0)
+        s.find_constraints()
+        s.compute_hessian_and_gradient()
+    it0, rel0 = g.solve_system(tol=loose)
+    it1, rel1 = g.refine_solve()
+    itr, relr = ref.solve_system()
+    b = ref.vector(2).ravel()
+    D = np.repeat(sc.dirichlet_mask, 3)
+    p = g.vector(1).ravel()
+    pr = ref.vector(1).ravel()
+    r = b - _spmv(ref, p)
+    r[D] = 0.0
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    print(f"precision {prec}: loose {it0} it ({rel0:.2e}), continued +{it1} it ({rel1:.2e}, FP64 {rel:.2e}); "
+          f"direct {itr} it ({relr:.2e})")
+    assert rel0 <= loose * 1.0001 and it0 > 0 and it1 > 0
+    assert rel <= tight * 1.0001
+    assert np.abs(p - pr).max() <= 1e4 * tight * np.abs(pr).max()
+    # a continuation needs the system the last solve solved
+    g.compute_hessian_and_gradient()
+    it_, rel_ = C.c_int(), C.c_double()
+    assert g.so.fipc_solve_refine(g.ctx, tight, 100, C.byref(it_), C.byref(rel_)) == 6  # FIPC_ERR_STATE
+    with pytest.raises(ValueError):
+        NewtonIPC(sc, pcg_tol=1e-6, pcg_tol_search=1e-8)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_inexact_search_directions_steps(prec):
+    """Whole steps with search directions at 1e-2 and the deciding direction
+    continued to 1e-12, against steps of the tight solver from the same state,
+    Newton tolerance tightened 1e3 times: every step converges
+    intersection-free and the positions agree within 1e-6 of the bbox
+    diagonal."""
+    sc = S.two_cubes(4)
+    sc.tol_rel *= 1e-3
+    g = NewtonIPC(sc, precision=prec, pcg_tol=1e-12, pcg_tol_search=1e-2)
+    ref = NewtonIPC(sc, precision=prec, pcg_tol=1e-12)
+    x = _deformed(sc, 0.004, 6)
+    g.set_state(x)
+    ref.set_state(x)
+    for f in range(6):
+        x0, v0 = g.x.copy(), g.v.copy()
+        ref.set_state(x0, v0)
+        st = g.step()
+        sr = ref.step()
+        assert st.converged and sr.converged and g.check_collision() == 0
+        err = np.abs(g.x - ref.x).max()
+        print(f"step {f}: newton {st.newton_iters} vs {sr.newton_iters}, pcg {sum(st.pcg_iters)} vs "
+              f"{sum(sr.pcg_iters)}, position difference {err / sc.bbox_diag:.3e} bbox")
         assert err <= 1e-6 * sc.bbox_diag, (f, err)
