@@ -221,35 +221,35 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) k
     });
 }
 
-// second pass, one thread per stored (upper) block: sum its contiguous run of
-// oriented tet blocks in the scratch (tets in index order), add the lumped
-// mass on diagonal blocks (compute_inertia, NEWTON_IPC.py:558-567), store the
-// FP64 values and, when valf != nullptr, their FP32 copy for the FP32 SpMV
-__global__ void k_gather_blocks(int nnzb, const int* __restrict__ bcptr, const double* __restrict__ tetH,
-                                const int* __restrict__ brow, const int* __restrict__ col,
-                                const double* __restrict__ m, double* __restrict__ val, float* __restrict__ valf)
+// second pass: sum each stored (upper) block's contiguous run of oriented tet
+// blocks in the scratch (tets in index order), add the lumped mass on diagonal
+// blocks (compute_inertia, NEWTON_IPC.py:558-567), store the FP64 values and,
+// when valf != nullptr, their FP32 copy for the FP32 SpMV.  Nine lanes per
+// block, one per entry, seven blocks per wavefront (lane 63 idles): the lanes
+// of a block read the 72 contiguous bytes of each record together, so a
+// load instruction touches a few cache lines instead of one per lane.  Each
+// entry is summed in record order, as by a single thread.
+constexpr int GATHER_BPW = 7;                       // blocks per wavefront
+constexpr int GATHER_BPG = GATHER_BPW * (TPB / 64); // blocks per workgroup
+static inline int gather_grid(long nnzb) { return (int)((nnzb + GATHER_BPG - 1) / GATHER_BPG); }
+__global__ void __launch_bounds__(TPB) k_gather_blocks(int nnzb, const int* __restrict__ bcptr,
+                                                       const double* __restrict__ tetH, const int* __restrict__ brow,
+                                                       const int* __restrict__ col, const double* __restrict__ m,
+                                                       double* __restrict__ val, float* __restrict__ valf)
 {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    if (lane >= 9 * GATHER_BPW) return;
+    const long b = ((long)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)) * GATHER_BPW + lane / 9;
+    const int e = lane % 9;
     if (b >= nnzb) return;
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double acc = 0.0;
+    const int t1 = bcptr[b + 1];
     // unrolled so that several records are in flight (same summation order)
 #pragma unroll 4
-    for (int t = bcptr[b]; t < bcptr[b + 1]; t++) {
-        const double* h = tetH + (size_t)t * 9;
-#pragma unroll
-        for (int k = 0; k < 9; k++) acc[k] += h[k];
-    }
-    int r = brow[b];
-    if (col[b] == r) {
-        double mi = m[r];
-        acc[0] += mi; acc[4] += mi; acc[8] += mi;
-    }
-#pragma unroll
-    for (int k = 0; k < 9; k++) val[(size_t)b * 9 + k] = acc[k];
-    if (valf) {
-#pragma unroll
-        for (int k = 0; k < 9; k++) valf[(size_t)b * 9 + k] = (float)acc[k];
-    }
+    for (int t = bcptr[b]; t < t1; t++) acc += tetH[(size_t)t * 9 + e];
+    if ((e == 0 || e == 4 || e == 8) && col[b] == brow[b]) acc += m[brow[b]];
+    val[(size_t)b * 9 + e] = acc;
+    if (valf) valf[(size_t)b * 9 + e] = (float)acc;
 }
 
 __global__ void k_gather_rhs(int nv, const int* __restrict__ vcptr, const int* __restrict__ vcent,
@@ -1386,7 +1386,7 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     STCHK(fipc_tm_end(c, TM_ELASTIC));
     STCHK(fipc_tm_begin(c, TM_GATHER));
     if (precision == 1) STCHK(ensure(c->valf, (size_t)c->nnzb * 9));
-    k_gather_blocks<<<nblk(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->tetH.p, c->brow.p, c->col.p, c->m.p,
+    k_gather_blocks<<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->tetH.p, c->brow.p, c->col.p, c->m.p,
                                                      c->val.p, precision == 1 ? c->valf.p : nullptr);
     k_gather_rhs<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->vcptr.p, c->vcent.p, c->tetg.p, c->m.p, c->x.p, c->xt.p,
                                                  c->dmask.p, c->rhs.p, c->cdiag.p);
