@@ -562,7 +562,9 @@ __global__ void k_precond(int nv, const double* __restrict__ val, const int* __r
 // lanes per block row: with 4, a wavefront covers 16 consecutive rows, whose
 // gathers share cache lines of p and of the value stream (faster than 2, 8 or
 // 16 lanes on the 1M-tet slab)
+#ifndef SPMV_G
 #define SPMV_G 4
+#endif
 // One launch computes A p: blocks [0, nb_c) run the contact rows of every
 // type (first, so that they overlap the BSR rows instead of trailing them),
 // the nb_bsr blocks after them the BSR rows (nb_c is a multiple of 8, so the
@@ -578,7 +580,9 @@ __global__ void k_precond(int nv, const double* __restrict__ val, const int* __r
 // BSR rows, p_v . (H_r p_r)_v for contact lanes (Dirichlet entries of the
 // PCG directions are zero, so this is the masked p.q).
 #define SPMV_ROWS (TPB / SPMV_G)
+#ifndef SPMV_MAXB
 #define SPMV_MAXB 1024
+#endif
 static inline int spmv_grid(int nv)
 {
     int groups = (nv + SPMV_ROWS - 1) / SPMV_ROWS;
@@ -791,7 +795,9 @@ __global__ void k_pcg_scalars(const double* part_rz, const double* part_bb, int 
 // test passed (done_at), state[1] = iterations run.  Both update kernels run
 // a grid-stride loop over vertices on at most PCG_MAXB blocks, so the
 // partials that the next kernel reduces stay few.
+#ifndef PCG_MAXB
 #define PCG_MAXB 512
+#endif
 static inline int pcg_grid(int nv) { return std::min(nblk(nv), PCG_MAXB); }
 // alpha = rz / pq (pq from the SpMV partials); x += alpha p; r -= alpha q;
 // z = Minv r; partials of r.z and r.r.  V / M: storage types of the

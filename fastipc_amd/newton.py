@@ -116,9 +116,13 @@ class NewtonIPC:
     # iteration reuses the constraints found after the last step of the line
     # search, and the next line search starts from the energy evaluated there
     # (same x, same constraints, deterministic reduction: the same bits).
+    # The collision test's result is kept the same way: the test after the
+    # energy backtracking repeats the one after the CCD step whenever no
+    # halving moved x in between.
     def _invalidate(self, x_changed: bool = True):
         if x_changed:
             self._cons_ok = False
+            self._hits = None
         self._E = None
 
     def _ensure_constraints(self):
@@ -225,9 +229,11 @@ class NewtonIPC:
         return (a.value, p) if parts else a.value
 
     def check_collision(self) -> int:
-        h = C.c_int()
-        _lib.call("fipc_check_collision", self.ctx, C.byref(h))
-        return h.value
+        if getattr(self, "_hits", None) is None:
+            h = C.c_int()
+            _lib.call("fipc_check_collision", self.ctx, C.byref(h))
+            self._hits = h.value
+        return self._hits
 
     def save_xPrev(self):
         _lib.call("fipc_save_prev", self.ctx)
