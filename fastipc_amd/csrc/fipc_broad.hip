@@ -455,9 +455,9 @@ static int find_once(fipc_ctx* c, int* counts, int* hflag)
     BoxFn qedges{c->x.p, nullptr, c->be.p, 1, c->dHat};
     VisitEE vee{c->x.p, c->x0.p, c->be.p, c->dHat, c->dHat2, sink};
     STCHK(run_query(c, 1, c->nbe, qedges, vee, TM_FIND));
-    HIPCHK(hipMemcpyAsync(counts, c->counters.p, NTYPES * 4, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(hflag, c->flag.p, 12, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
+    STCHK(d2h_async(c, counts, c->counters.p, NTYPES * 4));
+    STCHK(d2h_async(c, hflag, c->flag.p, 12));
+    STCHK(d2h_wait(c));
     return FIPC_OK;
 }
 
@@ -499,8 +499,8 @@ extern "C" int fipc_find_constraints(fipc_ctx* c, int* counts_out)
     }
     HIPCHK(hipGetLastError());
     int nu[2];
-    HIPCHK(hipMemcpyAsync(nu, c->nuniq.p, 8, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
+    STCHK(d2h_async(c, nu, c->nuniq.p, 8));
+    STCHK(d2h_wait(c));
     c->count[0] = nu[0];
     c->count[1] = nu[1];
     for (int t = 2; t < NTYPES; t++) c->count[t] = raw[t];
@@ -683,10 +683,10 @@ extern "C" int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* al
         if (inversion) STCHK(fipc_inversion_alpha(c, c->amin.p + 2));
         unsigned long long out[3];
         int hflag[3], hc[2];
-        HIPCHK(hipMemcpyAsync(out, c->amin.p, 24, hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipMemcpyAsync(hflag, c->flag.p, 12, hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipMemcpyAsync(hc, pc, 8, hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
+        STCHK(d2h_async(c, out, c->amin.p, 24));
+        STCHK(d2h_async(c, hflag, c->flag.p, 12));
+        STCHK(d2h_async(c, hc, pc, 8));
+        STCHK(d2h_wait(c));
         bool retry = false;
         STCHK(bp_check(c, hflag, &retry));
         int need = std::max(hc[0], hc[1]);
@@ -742,8 +742,8 @@ extern "C" int fipc_check_collision(fipc_ctx* c, int* hit)
         VisitCollide v{c->x.p, c->be.p, c->bt.p, c->flag.p + 1};
         STCHK(run_query(c, 1, c->nbt, tris, v, TM_FIND));
         int hflag[3];
-        HIPCHK(hipMemcpyAsync(hflag, c->flag.p, 12, hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
+        STCHK(d2h_async(c, hflag, c->flag.p, 12));
+        STCHK(d2h_wait(c));
         bool retry = false;
         STCHK(bp_check(c, hflag, &retry));
         if (retry) continue;

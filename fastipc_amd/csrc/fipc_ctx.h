@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 #include "fipc_math.h"
 #include "../../include/fastipc.h"
@@ -169,7 +170,52 @@ struct fipc_ctx {
     KernelTimer tm[FIPC_NTIMERS];
     std::vector<hipEvent_t> event_pool;
     bool timing = false;
+    // pinned host staging of the small device-to-host readbacks (counts,
+    // flags, PCG state and scalars, reduction partials): a copy into pageable
+    // memory is staged by the runtime and returns only once the stream has
+    // reached it, so two readbacks cost two round trips; into pinned memory
+    // they queue behind the kernels and one synchronisation collects them
+    char* pin = nullptr;
+    size_t pin_used = 0;
+    struct PendingCopy {
+        void* dst;
+        size_t off, n;
+    };
+    std::vector<PendingCopy> pend;
 };
+
+#define FIPC_PIN_BYTES (256u << 10)
+// queues the copy of n bytes from device memory src to host memory dst;
+// dst is written by the next d2h_wait (copies that do not fit the staging
+// area go straight to dst and complete by then too)
+static inline int d2h_async(fipc_ctx* c, void* dst, const void* src, size_t n)
+{
+    size_t off = (c->pin_used + 63) & ~(size_t)63;
+    if (!c->pin || off + n > FIPC_PIN_BYTES) {
+        HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, c->s));
+        return FIPC_OK;
+    }
+    hipError_t e = hipMemcpyAsync(c->pin + off, src, n, hipMemcpyDeviceToHost, c->s);
+    if (e != hipSuccess) {  // no stale destinations are left queued
+        c->pend.clear();
+        c->pin_used = 0;
+        HIPCHK(e);
+    }
+    c->pend.push_back({dst, off, n});
+    c->pin_used = off + n;
+    return FIPC_OK;
+}
+// waits for the stream and delivers the queued readbacks
+static inline int d2h_wait(fipc_ctx* c)
+{
+    hipError_t e = hipStreamSynchronize(c->s);
+    if (e == hipSuccess)
+        for (const auto& q : c->pend) memcpy(q.dst, c->pin + q.off, q.n);
+    c->pend.clear();
+    c->pin_used = 0;
+    HIPCHK(e);
+    return FIPC_OK;
+}
 
 int fipc_tm_begin(fipc_ctx* c, int which);
 int fipc_tm_end(fipc_ctx* c, int which);

@@ -1048,6 +1048,7 @@ extern "C" int fipc_create(fipc_ctx** out)
     if (ndev <= 0) return FIPC_ERR_HIP;
     fipc_ctx* c = new fipc_ctx();
     HIPCHK(hipStreamCreate(&c->s));
+    HIPCHK(hipHostMalloc((void**)&c->pin, FIPC_PIN_BYTES, hipHostMallocDefault));
     *out = c;
     return FIPC_OK;
 }
@@ -1056,6 +1057,7 @@ extern "C" int fipc_destroy(fipc_ctx* c)
 {
     if (!c) return FIPC_OK;
     (void)hipStreamSynchronize(c->s);
+    if (c->pin) (void)hipHostFree(c->pin);
     release(c->tets); release(c->bp); release(c->be); release(c->bt); release(c->dids);
     release(c->IB); release(c->vol); release(c->la); release(c->mu); release(c->m); release(c->dmask);
     release(c->x); release(c->x0); release(c->xn); release(c->xt); release(c->xprev); release(c->v);
@@ -1354,8 +1356,8 @@ extern "C" int fipc_residual(fipc_ctx* c, double* res)
     k_maxabs<<<nb, TPB, 0, c->s>>>(c->nv * 3, c->sol.p, c->part_a.p);
     HIPCHK(hipGetLastError());
     std::vector<double> h(nb);
-    HIPCHK(hipMemcpyAsync(h.data(), c->part_a.p, nb * 8, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
+    STCHK(d2h_async(c, h.data(), c->part_a.p, nb * 8));
+    STCHK(d2h_wait(c));
     double m = 0.0;
     for (double v : h) m = std::max(m, v);
     *res = m;
@@ -1490,8 +1492,8 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
     state[1] = 0;
     if (round > 0) {
         // the recomputed residual may already meet the tolerance
-        HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
+        STCHK(d2h_async(c, state, c->pcg_state.p, 8));
+        STCHK(d2h_wait(c));
         if (state[0] == -1) return FIPC_OK;
     }
     // the host checks convergence between chunks of iterations; the first
@@ -1510,13 +1512,13 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
             STCHK(fipc_tm_end(c, TM_PCGVEC));
         }
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
+        STCHK(d2h_async(c, state, c->pcg_state.p, 8));
+        STCHK(d2h_wait(c));
         if (state[0] != 0x7fffffff) break;
     }
     if (budget <= 0) {
-        HIPCHK(hipMemcpyAsync(state, c->pcg_state.p, 8, hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
+        STCHK(d2h_async(c, state, c->pcg_state.p, 8));
+        STCHK(d2h_wait(c));
     }
     return FIPC_OK;
 }
@@ -1575,8 +1577,8 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
     // (relres then reports the recursive residual of that round)
     if (!converged && state[0] != 0x7fffffff) converged = true;
     double sc[5];
-    HIPCHK(hipMemcpyAsync(sc, c->scal.p, 40, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
+    STCHK(d2h_async(c, sc, c->scal.p, 40));
+    STCHK(d2h_wait(c));
     double rel = sc[2] > 0 ? sqrt(sc[4] / sc[2]) : 0.0;
     c->last_pcg_iters = total;
     c->last_pcg_relres = rel;
@@ -1638,8 +1640,8 @@ extern "C" int fipc_energy(fipc_ctx* c, double* E, double* parts)
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_ENERGY));
     std::vector<double> h(total);
-    HIPCHK(hipMemcpyAsync(h.data(), ep, total * 8, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
+    STCHK(d2h_async(c, h.data(), ep, total * 8));
+    STCHK(d2h_wait(c));
     double ei = 0, ee = 0, eb = 0;
     for (int i = 0; i < nbv; i++) ei += h[i];
     for (int i = 0; i < nbt; i++) ee += h[nbv + i];
@@ -1674,8 +1676,8 @@ extern "C" int fipc_min_distance(fipc_ctx* c, double* d2min)
     }
     HIPCHK(hipGetLastError());
     unsigned long long r;
-    HIPCHK(hipMemcpyAsync(&r, c->amin.p, 8, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
+    STCHK(d2h_async(c, &r, c->amin.p, 8));
+    STCHK(d2h_wait(c));
     double d;
     memcpy(&d, &r, 8);
     *d2min = d;
