@@ -218,29 +218,103 @@ __global__ void k_gather_batch(int B, long nds, unsigned long long seed, int nin
     for (int k = 0; k < nout; k++) T[(long)i * nout + k] = dsT[r * nout + k];
 }
 
-// ---------------------------------------------------------------------------
-// Fused inference for the warm-start shape (16 -> 4 x 256 SiLU -> 3): a block
-// of 4 waves takes 128 rows through all five layers; the activations stay in
-// LDS as BF16 (the products' input precision) and never touch HBM.  Weights
-// are read from L2 as BF16 transposed per layer, Wt[n][k] (the last layer
-// padded to 32 rows of zeros), so a lane's B fragment (8 consecutive k of one
-// output column) is one 16-byte load.  A block has 8 waves; wave w owns output
-// columns 64 (w % 4) .. +64 (two 32-column MFMA tiles) for rows 64 (w / 4) ..
-// +64 (two 32-row tiles): each weight fragment feeds two MFMAs, each
-// activation fragment (from LDS) two, and 64 accumulators per lane leave
-// room for two waves per SIMD.  The accumulation order over k is the generic
-// GEMM's.
-// ---------------------------------------------------------------------------
 #define FI_ROWS 128
 #define FI_W 256
 #define FI_LDS (FI_W + 8)
-__global__ void __launch_bounds__(512) k_mlp_infer(int n, const float* __restrict__ X, const __bf16* __restrict__ Wt,
-                                                   const float* __restrict__ params, const long* __restrict__ boff,
-                                                   float* __restrict__ Y)
+// D[feature][row] += W[feature][k..k+15] . H[row][k..k+15] over the layer's
+// fan-in KIN for NF 32-feature tiles from f0 and two 32-row tiles from rbase.
+// The A operand is a weight fragment (8 consecutive k of one output feature,
+// one 16-byte load from the transposed BF16 weights), the B operand an
+// activation fragment (8 consecutive k of one row, one 16-byte LDS read);
+// the next k step's weight fragments are loaded before this step's products.
+template <int NF>
+__device__ __forceinline__ void fi_step(const bf16x8 (&a)[NF], const __bf16 (*H)[FI_LDS], int rbase, int lane, int k0,
+                                        floatx16 (&acc)[NF][2])
+{
+    bf16x8 b0 = *(const bf16x8*)&H[rbase + (lane & 31)][k0 + (lane >> 5) * 8];
+    bf16x8 b1 = *(const bf16x8*)&H[rbase + 32 + (lane & 31)][k0 + (lane >> 5) * 8];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        acc[f][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[f], b0, acc[f][0], 0, 0, 0);
+        acc[f][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[f], b1, acc[f][1], 0, 0, 0);
+    }
+}
+template <int KIN, int NF>
+__device__ __forceinline__ void fi_layer(const __bf16* __restrict__ W, const __bf16 (*H)[FI_LDS], int f0, int rbase,
+                                         int lane, floatx16 (&acc)[NF][2])
+{
+    static_assert(KIN == 16 || KIN % 32 == 0, "fan-in");
+    const __bf16* wp[NF];
+    bf16x8 a[NF], an[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        wp[f] = W + (long)(f0 + f * 32 + (lane & 31)) * KIN + (lane >> 5) * 8;
+        a[f] = *(const bf16x8*)wp[f];
+    }
+    if (KIN == 16) {
+        fi_step<NF>(a, H, rbase, lane, 0, acc);
+        return;
+    }
+    // two k steps per pass in alternating registers: the fragments of the
+    // next step are in flight during this step's products (the last pass
+    // reloads its own last step rather than read past the row)
+    for (int k0 = 0; k0 < KIN; k0 += 32) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) an[f] = *(const bf16x8*)(wp[f] + k0 + 16);
+        fi_step<NF>(a, H, rbase, lane, k0, acc);
+        const int kn = k0 + 32 < KIN ? k0 + 32 : KIN - 16;
+#pragma unroll
+        for (int f = 0; f < NF; f++) a[f] = *(const bf16x8*)(wp[f] + kn);
+        fi_step<NF>(an, H, rbase, lane, k0 + 16, acc);
+    }
+}
+__device__ __forceinline__ unsigned int bf16_bits(float v)
+{
+    return (unsigned int)__builtin_bit_cast(unsigned short, (__bf16)v);
+}
+// hidden-layer epilogue: bias, SiLU, BF16 into the activation tile.  Lane l
+// holds features f + 8 (j / 4) + 4 (l / 32) + j % 4 of row (l % 32) of each
+// tile, so each group of four j is four consecutive features of one row:
+// one 8-byte LDS store
+__device__ __forceinline__ void fi_silu_store(const floatx16 (&acc)[2][2], const float* __restrict__ bias,
+                                              __bf16 (*H)[FI_LDS], int f0, int rbase, int lane)
+{
+#pragma unroll
+    for (int ft = 0; ft < 2; ft++)
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const int m = f0 + ft * 32 + 8 * g + 4 * (lane >> 5);
+            const float b0 = bias[m], b1 = bias[m + 1], b2 = bias[m + 2], b3 = bias[m + 3];
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++) {
+                float z0 = acc[ft][rt][4 * g] + b0, z1 = acc[ft][rt][4 * g + 1] + b1;
+                float z2 = acc[ft][rt][4 * g + 2] + b2, z3 = acc[ft][rt][4 * g + 3] + b3;
+                uint2 pk;
+                pk.x = bf16_bits(z0 * sigm(z0)) | (bf16_bits(z1 * sigm(z1)) << 16);
+                pk.y = bf16_bits(z2 * sigm(z2)) | (bf16_bits(z3 * sigm(z3)) << 16);
+                *(uint2*)&H[rbase + rt * 32 + (lane & 31)][m] = pk;
+            }
+        }
+}
+// ---------------------------------------------------------------------------
+// Fused inference for the warm-start shape (16 -> 4 x 256 SiLU -> 3): a block
+// of 8 waves takes 128 rows through all five layers; the activations stay in
+// LDS as BF16 (the products' input precision) and never touch HBM.  Weights
+// are read from L2 as BF16 transposed per layer, Wt[n][k] (the last layer
+// padded to 32 rows of zeros).  Wave w owns output features 64 (w % 4) .. +64
+// (two 32-feature MFMA tiles) for rows 64 (w / 4) .. +64 (two 32-row tiles):
+// each weight fragment feeds two MFMAs, each activation fragment two.  At
+// most 128 registers per lane (4 waves per SIMD, two blocks per CU, which the
+// 66 KB activation tile allows); the accumulation order over k is the
+// generic GEMM's.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+k_mlp_infer(int n, const float* __restrict__ X, const __bf16* __restrict__ Wt, const float* __restrict__ params,
+            const long* __restrict__ boff, float* __restrict__ Y)
 {
     __shared__ __attribute__((aligned(16))) __bf16 H[FI_ROWS][FI_LDS];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int cq = w & 3, rh = w >> 2;
+    const int f0 = (w & 3) * 64, rbase = (w >> 2) * 64;
     const long r0 = (long)blockIdx.x * FI_ROWS;
     // input tile: 128 rows x 16 features
     for (int e = tid; e < FI_ROWS * 16; e += blockDim.x) {
@@ -250,56 +324,39 @@ __global__ void __launch_bounds__(512) k_mlp_infer(int n, const float* __restric
     }
     __syncthreads();
     const __bf16* W = Wt;
-    for (int l = 0; l < 5; l++) {
-        const int fi = l == 0 ? 16 : FI_W;
-        const int ntile = l == 4 ? (cq == 0 ? 1 : 0) : 2;
+    for (int l = 0; l < 4; l++) {
         floatx16 acc[2][2];
 #pragma unroll
-        for (int rt = 0; rt < 2; rt++)
-#pragma unroll
-            for (int t = 0; t < 2; t++)
-#pragma unroll
-                for (int j = 0; j < 16; j++) acc[rt][t][j] = 0.f;
-        if (ntile > 0) {
-            for (int k0 = 0; k0 < fi; k0 += 16) {
-                bf16x8 b[2];
-#pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    int col = cq * 64 + t * 32 + (lane & 31);
-                    if (t < ntile) b[t] = *(const bf16x8*)&W[(long)col * fi + k0 + (lane >> 5) * 8];
-                }
-#pragma unroll
-                for (int rt = 0; rt < 2; rt++) {
-                    bf16x8 a = *(const bf16x8*)&H[rh * 64 + rt * 32 + (lane & 31)][k0 + (lane >> 5) * 8];
-#pragma unroll
-                    for (int t = 0; t < 2; t++)
-                        if (t < ntile) acc[rt][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[t], acc[rt][t], 0, 0, 0);
-                }
-            }
-        }
-        __syncthreads();  // every wave has read this layer's input
-        const float* bias = params + boff[l];
-#pragma unroll
-        for (int t = 0; t < 2; t++) {
-            if (t >= ntile) continue;
-            int col = cq * 64 + t * 32 + (lane & 31);
-            float bc = (l < 4 || col < 3) ? bias[col] : 0.f;
+        for (int f = 0; f < 2; f++)
 #pragma unroll
             for (int rt = 0; rt < 2; rt++)
 #pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    int row = rh * 64 + rt * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-                    float z = acc[rt][t][j] + bc;
-                    if (l < 4) {
-                        H[row][col] = (__bf16)(z * sigm(z));
-                    } else if (col < 3) {
-                        long gr = r0 + row;
-                        if (gr < n) Y[gr * 3 + col] = z;
-                    }
-                }
-        }
+                for (int j = 0; j < 16; j++) acc[f][rt][j] = 0.f;
+        if (l == 0) fi_layer<16, 2>(W, H, f0, rbase, lane, acc);
+        else fi_layer<FI_W, 2>(W, H, f0, rbase, lane, acc);
+        __syncthreads();  // every wave has read this layer's input
+        fi_silu_store(acc, params + boff[l], H, f0, rbase, lane);
         __syncthreads();
-        W += (long)(l == 4 ? 32 : FI_W) * fi;
+        W += (long)FI_W * (l == 0 ? 16 : FI_W);
+    }
+    // output layer: features 0..2 of the first 32-feature tile
+    if (f0 == 0) {
+        floatx16 acc[1][2];
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+            for (int j = 0; j < 16; j++) acc[0][rt][j] = 0.f;
+        fi_layer<FI_W, 1>(W, H, 0, rbase, lane, acc);
+        if (lane < 32) {
+            const float* bias = params + boff[4];
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++) {
+                long gr = r0 + rbase + rt * 32 + lane;
+                if (gr < n)
+#pragma unroll
+                    for (int i = 0; i < 3; i++) Y[gr * 3 + i] = acc[0][rt][i] + bias[i];
+            }
+        }
     }
 }
 // Wt (BF16, transposed, last layer padded to 32 rows) from the FP32 parameters
