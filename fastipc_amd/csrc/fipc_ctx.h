@@ -167,6 +167,9 @@ struct fipc_ctx {
     // fipc_assemble bumps sys_version; fipc_solve records the version it
     // solved, which fipc_solve_refine requires
     unsigned long long sys_version = 1, solve_version = 0;
+    // the last solve ended on an FP32-mode restart (r, z, p and the scalars
+    // at its solution, FP64 residual) that a continuation can iterate from
+    bool restart_ready = false;
     KernelTimer tm[FIPC_NTIMERS];
     std::vector<hipEvent_t> event_pool;
     bool timing = false;

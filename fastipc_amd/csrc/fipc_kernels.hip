@@ -790,6 +790,16 @@ __global__ void k_pcg_scalars(const double* part_rz, const double* part_bb, int 
         state[1] = 0;
     }
 }
+// the convergence test of a kept restart (scal[4] = |r|^2 of its FP64
+// residual) against a new tolerance
+__global__ void k_pcg_retest(const double* scal, int* state, double tol2)
+{
+    if (threadIdx.x == 0) {
+        double bb = scal[4];
+        state[0] = (bb <= tol2 * scal[2] || !(bb == bb)) ? -1 : 0x7fffffff;
+        state[1] = 0;
+    }
+}
 // scal[0|1] = r.z of the current / next iteration (by parity), scal[2] =
 // |r0|^2, scal[4] = |r|^2, scal[5] = p.q; state[0] = iteration at which the
 // test passed (done_at), state[1] = iterations run.  Both update kernels run
@@ -1290,6 +1300,7 @@ extern "C" int fipc_set_vector(fipc_ctx* c, int which, const double* in)
     HIPCHK(hipMemcpyAsync(d->p, in, (size_t)c->nv * 24, hipMemcpyHostToDevice, c->s));
     if (d == &c->x) c->x_version++;
     if (d != &c->sol) c->sys_version++;   // a continued solve needs the system it solved
+    c->restart_ready = false;
     HIPCHK(hipStreamSynchronize(c->s));
     return FIPC_OK;
 }
@@ -1477,24 +1488,35 @@ extern "C" int fipc_spmv(fipc_ctx* c, const double* x, double* y)
 // Hessian; FP32 in the FP32 Hessian's refinement rounds)
 template <class V, class M>
 static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* q, int round, int budget,
-                     double tol2, int hint, int state[2])
+                     double tol2, int hint, int state[2], bool reuse, bool* init_only)
 {
     const int nv = c->nv, nbv = nblk(nv), nsp = spmv_nparts(c), nu = pcg_grid(nv), chunk = 8;
-    // r = b - A x with the FP64 blocks (q + qc), z = Minv r, p = z
-    STCHK(spmv<double>(c, 0, c->sol.p, c->q.p, c->qc.p, nullptr, nullptr, 0));
-    k_pcg_init<V><<<nbv, TPB, 0, c->s>>>(nv, c->rhs.p, c->q.p, c->qc.p, c->dmask.p, c->Minv.p, r, z, p, c->part_a.p,
-                                         c->part_b.p);
-    HIPCHK(hipGetLastError());
-    k_pcg_scalars<<<1, TPB, 0, c->s>>>(c->part_a.p, c->part_b.p, nbv, c->scal.p, c->pcg_state.p, round > 0, tol2);
-    HIPCHK(hipGetLastError());
     int it = 0;
     state[0] = 0x7fffffff;
     state[1] = 0;
-    if (round > 0) {
-        // the recomputed residual may already meet the tolerance
-        STCHK(d2h_async(c, state, c->pcg_state.p, 8));
-        STCHK(d2h_wait(c));
-        if (state[0] == -1) return FIPC_OK;
+    *init_only = false;
+    if (reuse) {
+        // r, z, p and the scalars still hold the restart at the current
+        // solution that ended the last solve; only the test changes
+        k_pcg_retest<<<1, 64, 0, c->s>>>(c->scal.p, c->pcg_state.p, tol2);
+        HIPCHK(hipGetLastError());
+    } else {
+        // r = b - A x with the FP64 blocks (q + qc), z = Minv r, p = z
+        STCHK(spmv<double>(c, 0, c->sol.p, c->q.p, c->qc.p, nullptr, nullptr, 0));
+        k_pcg_init<V><<<nbv, TPB, 0, c->s>>>(nv, c->rhs.p, c->q.p, c->qc.p, c->dmask.p, c->Minv.p, r, z, p,
+                                             c->part_a.p, c->part_b.p);
+        HIPCHK(hipGetLastError());
+        k_pcg_scalars<<<1, TPB, 0, c->s>>>(c->part_a.p, c->part_b.p, nbv, c->scal.p, c->pcg_state.p, round > 0, tol2);
+        HIPCHK(hipGetLastError());
+        if (round > 0) {
+            // the recomputed residual may already meet the tolerance
+            STCHK(d2h_async(c, state, c->pcg_state.p, 8));
+            STCHK(d2h_wait(c));
+            if (state[0] == -1) {
+                *init_only = true;
+                return FIPC_OK;
+            }
+        }
     }
     // the host checks convergence between chunks of iterations; the first
     // chunk is sized from a previous solve of the same kind (hint), whose
@@ -1545,7 +1567,11 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
     }
     const int rounds = prec == 1 ? 4 : 1;
     int total = 0, state[2] = {0, 0};
-    bool converged = false;
+    bool converged = false, init_only = false;
+    // a continuation in FP32 mode starts from the restart that ended the
+    // solve it continues, when there is one (its FP64 residual is current)
+    bool reuse = first_round > 0 && prec == 1 && c->restart_ready;
+    c->restart_ready = false;
     for (int round = first_round; round < first_round + rounds; round++) {
         if (round == 0) {
             // x0 = prescribed Dirichlet increments
@@ -1558,12 +1584,13 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
         // first chunk: the last first round's count, or that of the last
         // continuation (fipc_solve_refine) for its first round
         int hint = round == 0 ? c->pcg_hint : (round == first_round ? c->refine_hint : 0);
+        const bool ru = reuse && round == first_round;
         if (prec == 1)
             STCHK((pcg_round<float, float>(c, prec, c->Minvf.p, c->rf.p, c->zf.p, c->pf.p, c->qf.p, round, budget,
-                                           tol2, hint, state)));
+                                           tol2, hint, state, ru, &init_only)));
         else
             STCHK((pcg_round<double, double>(c, prec, c->Minv.p, c->r.p, c->z.p, c->pp.p, c->q.p, round, budget,
-                                             tol2, hint, state)));
+                                             tol2, hint, state, ru, &init_only)));
         if (state[0] == -1 && round > 0) { converged = true; break; }
         int ran = state[0] == -1 ? 0 : state[1];
         total += ran;
@@ -1576,6 +1603,9 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
     // refinement rounds exhausted after a converged FP32 round: accept it
     // (relres then reports the recursive residual of that round)
     if (!converged && state[0] != 0x7fffffff) converged = true;
+    // a solve that ended on a restart whose FP64 residual met the tolerance
+    // leaves that restart for a continuation
+    c->restart_ready = prec == 1 && init_only;
     double sc[5];
     STCHK(d2h_async(c, sc, c->scal.p, 40));
     STCHK(d2h_wait(c));
