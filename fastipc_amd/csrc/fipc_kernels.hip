@@ -252,26 +252,37 @@ __global__ void __launch_bounds__(TPB) k_gather_blocks(int nnzb, const int* __re
     if (valf) valf[(size_t)b * 9 + e] = (float)acc;
 }
 
-__global__ void k_gather_rhs(int nv, const int* __restrict__ vcptr, const int* __restrict__ vcent,
-                             const double* __restrict__ tetg, const double* __restrict__ m, const double* __restrict__ x,
-                             const double* __restrict__ xt, const unsigned char* __restrict__ dmask, double* rhs,
-                             double* cdiag)
+// the gradient: inertia term plus the vertex's tet contributions in order
+// (rhs = -g; rhs[D] = 0, NEWTON_IPC.py:772), and the contact diagonal blocks
+// cleared for k_contact.  Three lanes per vertex, one per component, 21
+// vertices per wavefront (lane 63 idles): the lanes of a vertex read each
+// 24-byte contribution together.  Each component is summed in the order of a
+// single thread.
+constexpr int RHS_VPW = 21;                     // vertices per wavefront
+constexpr int RHS_VPG = RHS_VPW * (TPB / 64);   // vertices per workgroup
+static inline int rhs_grid(long nv) { return (int)((nv + RHS_VPG - 1) / RHS_VPG); }
+__global__ void __launch_bounds__(TPB) k_gather_rhs(int nv, const int* __restrict__ vcptr,
+                                                    const int* __restrict__ vcent, const double* __restrict__ tetg,
+                                                    const double* __restrict__ m, const double* __restrict__ x,
+                                                    const double* __restrict__ xt,
+                                                    const unsigned char* __restrict__ dmask, double* rhs,
+                                                    double* cdiag)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    if (lane >= 3 * RHS_VPW) return;
+    const long i = ((long)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)) * RHS_VPW + lane / 3;
+    const int d = lane % 3;
     if (i >= nv) return;
-    for (int k = 0; k < 9; k++) cdiag[9 * i + k] = 0.0;
-    if (dmask[i]) {  // rhs[D] = 0 (NEWTON_IPC.py:772)
-        rhs[3 * i] = rhs[3 * i + 1] = rhs[3 * i + 2] = 0.0;
+    for (int k = 0; k < 3; k++) cdiag[9 * i + 3 * d + k] = 0.0;
+    if (dmask[i]) {
+        rhs[3 * i + d] = 0.0;
         return;
     }
-    double mi = m[i];
-    double r0 = -mi * (x[3 * i] - xt[3 * i]), r1 = -mi * (x[3 * i + 1] - xt[3 * i + 1]), r2 = -mi * (x[3 * i + 2] - xt[3 * i + 2]);
+    double r = -m[i] * (x[3 * i + d] - xt[3 * i + d]);
+    const int t1 = vcptr[i + 1];
 #pragma unroll 4
-    for (int t = vcptr[i]; t < vcptr[i + 1]; t++) {
-        const double* g = tetg + (size_t)vcent[t] * 3;
-        r0 -= g[0]; r1 -= g[1]; r2 -= g[2];
-    }
-    rhs[3 * i] = r0; rhs[3 * i + 1] = r1; rhs[3 * i + 2] = r2;
+    for (int t = vcptr[i]; t < t1; t++) r -= tetg[(size_t)vcent[t] * 3 + d];
+    rhs[3 * i + d] = r;
 }
 
 // new position of position k after a round of the round-robin ("circle")
@@ -734,28 +745,42 @@ __global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ ro
 // initial guess and a zero residual afterwards.
 // ---------------------------------------------------------------------------
 // x = p0 (prescribed on Dirichlet dofs, 0 elsewhere)
-__global__ void k_pcg_x0(int nv, const double* xt, const double* xcur, const unsigned char* dmask, double* sol, double* qc)
+// x0 = prescribed Dirichlet increments (0 elsewhere); *nonzero (set to -1
+// beforehand) becomes 0 if any increment is non-zero, so that the product
+// A x0 and its use can be skipped when x0 = 0 (then b - A x0 = b exactly)
+__global__ void k_pcg_x0(int nv, const double* xt, const double* xcur, const unsigned char* dmask, double* sol,
+                         double* qc, int* nonzero)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nv) return;
-    for (int d = 0; d < 3; d++) {
-        sol[3 * i + d] = dmask[i] ? xt[3 * i + d] - xcur[3 * i + d] : 0.0;
-        qc[3 * i + d] = 0.0;
+    bool nz = false;
+    if (i < nv) {
+        for (int d = 0; d < 3; d++) {
+            double v = dmask[i] ? xt[3 * i + d] - xcur[3 * i + d] : 0.0;
+            sol[3 * i + d] = v;
+            qc[3 * i + d] = 0.0;
+            nz |= v != 0.0;
+        }
     }
+    if (__ballot(nz) != 0 && (threadIdx.x & 63) == 0) nonzero[0] = 0;
 }
 // r = b - (q + qc) (= b - A x, from FP64 products); z = Minv r; p = z; qc
 // cleared for the iterations; partials of r.z and r.r.  V is the type the
 // iterations keep r, z, p in (float for the FP32 Hessian's refinement rounds).
+// x0zero (may be null): -1 when x = 0, whose product was skipped (q, qc
+// not written: r = b)
 template <class V>
 __global__ void k_pcg_init(int nv, const double* b, const double* q, double* qc, const unsigned char* dmask,
-                           const double* Minv, V* r, V* z, V* p, double* part_rz, double* part_bb)
+                           const double* Minv, V* r, V* z, V* p, double* part_rz, double* part_bb,
+                           const int* x0zero)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0, bb = 0.0;
+    const bool skip = x0zero && x0zero[0] == -1;
     if (i < nv) {
         double ri[3], zi[3];
         bool fixed = dmask[i];
-        for (int d = 0; d < 3; d++) ri[d] = fixed ? 0.0 : b[3 * i + d] - (q[3 * i + d] + qc[3 * i + d]);
+        for (int d = 0; d < 3; d++)
+            ri[d] = fixed ? 0.0 : (skip ? b[3 * i + d] : b[3 * i + d] - (q[3 * i + d] + qc[3 * i + d]));
         const double* M = Minv + 9 * i;
         for (int d = 0; d < 3; d++) zi[d] = fixed ? 0.0 : M[3 * d] * ri[0] + M[3 * d + 1] * ri[1] + M[3 * d + 2] * ri[2];
         for (int d = 0; d < 3; d++) {
@@ -805,6 +830,14 @@ __global__ void k_pcg_retest(const double* scal, int* state, double tol2)
 // test passed (done_at), state[1] = iterations run.  Both update kernels run
 // a grid-stride loop over vertices on at most PCG_MAXB blocks, so the
 // partials that the next kernel reduces stay few.
+// iterations per host convergence check after the first chunk, and how many
+// iterations short of the hint the first chunk stops
+#ifndef PCG_CHUNK
+#define PCG_CHUNK 8
+#endif
+#ifndef PCG_FIRST_SHORT
+#define PCG_FIRST_SHORT 4
+#endif
 #ifndef PCG_MAXB
 #define PCG_MAXB 512
 #endif
@@ -1082,6 +1115,7 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->Minvf); release(c->rf); release(c->zf); release(c->pf); release(c->qf);
     release(c->trow); release(c->tval);
     release(c->part_a); release(c->part_b); release(c->part_c); release(c->scal); release(c->pcg_state);
+    release(c->x0flag);
     release(c->epart); release(c->amin); release(c->flag);
     for (Grid& g : c->grid) {
         release(g.cnt); release(g.ekey); release(g.eitem); release(g.bstart);
@@ -1407,7 +1441,7 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     if (precision == 1) STCHK(ensure(c->valf, (size_t)c->nnzb * 9));
     k_gather_blocks<<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->tetH.p, c->brow.p, c->col.p, c->m.p,
                                                      c->val.p, precision == 1 ? c->valf.p : nullptr);
-    k_gather_rhs<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->vcptr.p, c->vcent.p, c->tetg.p, c->m.p, c->x.p, c->xt.p,
+    k_gather_rhs<<<rhs_grid(c->nv), TPB, 0, c->s>>>(c->nv, c->vcptr.p, c->vcent.p, c->tetg.p, c->m.p, c->x.p, c->xt.p,
                                                  c->dmask.p, c->rhs.p, c->cdiag.p);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_GATHER));
@@ -1490,7 +1524,7 @@ template <class V, class M>
 static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* q, int round, int budget,
                      double tol2, int hint, int state[2], bool reuse, bool* init_only)
 {
-    const int nv = c->nv, nbv = nblk(nv), nsp = spmv_nparts(c), nu = pcg_grid(nv), chunk = 8;
+    const int nv = c->nv, nbv = nblk(nv), nsp = spmv_nparts(c), nu = pcg_grid(nv), chunk = PCG_CHUNK;
     int it = 0;
     state[0] = 0x7fffffff;
     state[1] = 0;
@@ -1501,10 +1535,12 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
         k_pcg_retest<<<1, 64, 0, c->s>>>(c->scal.p, c->pcg_state.p, tol2);
         HIPCHK(hipGetLastError());
     } else {
-        // r = b - A x with the FP64 blocks (q + qc), z = Minv r, p = z
-        STCHK(spmv<double>(c, 0, c->sol.p, c->q.p, c->qc.p, nullptr, nullptr, 0));
+        // r = b - A x with the FP64 blocks (q + qc), z = Minv r, p = z; in
+        // round 0 the product exits at once when x0 = 0 (the flag is -1)
+        const int* x0zero = round == 0 ? c->x0flag.p : nullptr;
+        STCHK(spmv<double>(c, 0, c->sol.p, c->q.p, c->qc.p, nullptr, x0zero, 0));
         k_pcg_init<V><<<nbv, TPB, 0, c->s>>>(nv, c->rhs.p, c->q.p, c->qc.p, c->dmask.p, c->Minv.p, r, z, p,
-                                             c->part_a.p, c->part_b.p);
+                                             c->part_a.p, c->part_b.p, x0zero);
         HIPCHK(hipGetLastError());
         k_pcg_scalars<<<1, TPB, 0, c->s>>>(c->part_a.p, c->part_b.p, nbv, c->scal.p, c->pcg_state.p, round > 0, tol2);
         HIPCHK(hipGetLastError());
@@ -1521,7 +1557,7 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
     // the host checks convergence between chunks of iterations; the first
     // chunk is sized from a previous solve of the same kind (hint), whose
     // count is usually close
-    int first = std::max(chunk, hint - chunk / 2);
+    int first = std::max(chunk, hint - PCG_FIRST_SHORT);
     while (it < budget) {
         int end = std::min(budget, it + (it == 0 ? first : chunk));
         for (; it < end; it++) {
@@ -1575,7 +1611,9 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
     for (int round = first_round; round < first_round + rounds; round++) {
         if (round == 0) {
             // x0 = prescribed Dirichlet increments
-            k_pcg_x0<<<nbv, TPB, 0, c->s>>>(nv, c->xt.p, c->x.p, c->dmask.p, c->sol.p, c->qc.p);
+            STCHK(ensure(c->x0flag, 1));
+            HIPCHK(hipMemsetAsync(c->x0flag.p, 0xff, 4, c->s));
+            k_pcg_x0<<<nbv, TPB, 0, c->s>>>(nv, c->xt.p, c->x.p, c->dmask.p, c->sol.p, c->qc.p, c->x0flag.p);
             HIPCHK(hipGetLastError());
         } else {
             HIPCHK(hipMemsetAsync(c->qc.p, 0, (size_t)nv * 24, c->s));
