@@ -138,6 +138,7 @@ struct fipc_ctx {
     DVec<double> tval;
     DVec<double> part_a, part_b, part_c, scal;
     DVec<int> x0flag;      // -1 when the PCG start x0 (Dirichlet increments) is zero
+    hipEvent_t ws_done = nullptr;  // end of the last warm-start prediction on this stream
     DVec<int> pcg_state;   // [0] done_at, [1] iterations
     // reductions for energy / residual / ccd
     DVec<double> epart;

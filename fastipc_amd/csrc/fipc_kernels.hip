@@ -1101,6 +1101,7 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     if (!c) return FIPC_OK;
     (void)hipStreamSynchronize(c->s);
     if (c->pin) (void)hipHostFree(c->pin);
+    if (c->ws_done) (void)hipEventDestroy(c->ws_done);
     release(c->tets); release(c->bp); release(c->be); release(c->bt); release(c->dids);
     release(c->IB); release(c->vol); release(c->la); release(c->mu); release(c->m); release(c->dmask);
     release(c->x); release(c->x0); release(c->xn); release(c->xt); release(c->xprev); release(c->v);

@@ -226,7 +226,12 @@ extern "C" int fipc_ws_predict(fipc_ctx* c, fipc_mlp* m, double scale)
     k_ws_apply<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, Y, 3, scale, c->x.p, c->xt.p, c->dprev.p, c->dmask.p, c->sol.p);
     c->restart_ready = false;  // sol changed
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->s));
+    // no host wait: later work on the network's own stream (training,
+    // parameter changes, destruction) waits for this stream's use of its
+    // buffers instead
+    if (!c->ws_done) HIPCHK(hipEventCreateWithFlags(&c->ws_done, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(c->ws_done, c->s));
+    HIPCHK(hipStreamWaitEvent(mlp_stream(m), c->ws_done, 0));
     return FIPC_OK;
 }
 

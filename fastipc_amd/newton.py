@@ -342,9 +342,11 @@ class NewtonIPC:
         incremental potential is not below its value at x_n; if no halving
         succeeds, x = x_n."""
         self._ensure_constraints()
-        E0 = self._energy()
         self.save_xPrev()
+        # the prediction is queued first: E0 (at x_n; the prediction writes
+        # only the direction) then waits for both at one synchronisation
         dx = fn(self)
+        E0 = self._energy()
         if dx is None or dx is False:
             return False
         if dx is not True:
