@@ -171,3 +171,24 @@ def test_inexact_search_directions_steps(prec):
         print(f"step {f}: newton {st.newton_iters} vs {sr.newton_iters}, pcg {sum(st.pcg_iters)} vs "
               f"{sum(sr.pcg_iters)}, position difference {err / sc.bbox_diag:.3e} bbox")
         assert err <= 1e-6 * sc.bbox_diag, (f, err)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_residual_is_max_abs_direction(prec):
+    """output_residual (max |p|, NEWTON_IPC.py:805-812) equals the maximum of
+    the direction read back, bit for bit: after a solve (measured by the PCG
+    iterations), after a continuation, and after the direction is overwritten
+    from the host (measured by its own reduction)."""
+    sc = S.two_cubes(5)
+    g = NewtonIPC(sc, precision=prec, pcg_tol=1e-9, pcg_tol_search=1e-2)
+    g.set_state(_deformed(sc, 0.004, 7))
+    g.compute_xn_and_xTilde(0.0)
+    g.find_constraints()
+    g.compute_hessian_and_gradient()
+    g.solve_system(tol=1e-2)
+    assert g.output_residual() == np.abs(g.vector(1)).max()
+    g.refine_solve()
+    assert g.output_residual() == np.abs(g.vector(1)).max()
+    p = np.random.default_rng(8).normal(size=(g.nv, 3))
+    g.set_vector(1, p)
+    assert g.output_residual() == np.abs(p).max()
