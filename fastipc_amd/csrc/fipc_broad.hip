@@ -505,6 +505,7 @@ extern "C" int fipc_find_constraints(fipc_ctx* c, int* counts_out)
     c->count[1] = nu[1];
     for (int t = 2; t < NTYPES; t++) c->count[t] = raw[t];
     c->sys_version++;   // the contact rows of the SpMV changed
+    c->cons_version++;  // and those of the barrier energy
     for (int t = 0; t < NTYPES; t++) {
         int N = 3 * (t == 0 ? 2 : (t == 1 ? 3 : 4));
         STCHK(ensure(c->cH[t], (size_t)std::max(c->count[t], 1) * N * N));

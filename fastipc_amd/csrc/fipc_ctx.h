@@ -139,6 +139,12 @@ struct fipc_ctx {
     DVec<double> part_a, part_b, part_c, scal;
     DVec<int> x0flag;      // -1 when the PCG start x0 (Dirichlet increments) is zero
     hipEvent_t ws_done = nullptr;  // end of the last warm-start prediction on this stream
+    // elastic and barrier energy of the last evaluation, valid for its x
+    // version and constraint-set version (cons_version: bumped by every
+    // constraint search); parameter changes clear e_ok
+    unsigned long long cons_version = 1, e_xv = 0, e_cv = 0;
+    double e_el = 0.0, e_bar = 0.0;
+    bool e_ok = false;
     DVec<int> pcg_state;   // [0] done_at, [1] iterations
     // reductions for energy / residual / ccd
     DVec<double> epart;

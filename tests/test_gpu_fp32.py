@@ -192,3 +192,24 @@ def test_residual_is_max_abs_direction(prec):
     p = np.random.default_rng(8).normal(size=(g.nv, 3))
     g.set_vector(1, p)
     assert g.output_residual() == np.abs(p).max()
+
+
+def test_energy_reuse_matches_full_evaluation():
+    """After xTilde moves (compute_xn_and_xTilde) at an unchanged x and
+    constraint set, compute_energy re-evaluates only the inertia term; the
+    result equals a full evaluation (forced by rewriting the same x) bit for
+    bit, and so do its parts."""
+    sc = S.two_cubes(4)
+    g = NewtonIPC(sc)
+    x = _deformed(sc, 0.003, 9)
+    g.set_state(x, np.full_like(x, 0.01))
+    g.compute_xn_and_xTilde(0.0)
+    g.find_constraints()
+    e1 = g.compute_energy(parts=True)
+    g.compute_xn_and_xTilde(sc.dt)      # new xTilde, same x and constraints
+    e2 = g.compute_energy(parts=True)
+    g.set_state(g.x)                     # same x, new x version: full evaluation
+    g.find_constraints()
+    e3 = g.compute_energy(parts=True)
+    assert e2[0] == e3[0] and np.array_equal(e2[1], e3[1])
+    assert e1[1][1] == e3[1][1] and e1[1][2] == e3[1][2] and e1[0] != e3[0]
