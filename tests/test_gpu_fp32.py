@@ -206,6 +206,7 @@ def test_energy_reuse_matches_full_evaluation():
     g.compute_xn_and_xTilde(0.0)
     g.find_constraints()
     e1 = g.compute_energy(parts=True)
+    g.set_state(v=np.full_like(x, -0.02))  # new velocity only: x version unchanged
     g.compute_xn_and_xTilde(sc.dt)      # new xTilde, same x and constraints
     e2 = g.compute_energy(parts=True)
     g.set_state(g.x)                     # same x, new x version: full evaluation
