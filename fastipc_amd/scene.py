@@ -207,7 +207,7 @@ class Scene:
         """(rows, positions): the Dirichlet vertices (int32, ascending) and
         their prescribed positions at time ``t``, shape (len(rows), 3)."""
         mask = self.dirichlet_mask
-        key = (id(mask), int(mask.sum()))
+        key = (id(mask), int(np.count_nonzero(mask)))
         if getattr(self, "_drows_key", None) != key:
             self._drows = np.flatnonzero(mask).astype(np.int32)
             self._drows_key = key
