@@ -832,6 +832,11 @@ __global__ void k_pcg_retest(const double* scal, int* state, double tol2)
 // partials that the next kernel reduces stay few.
 // iterations per host convergence check after the first chunk, and how many
 // iterations short of the hint the first chunk stops
+// the FP32 iterations stop at this fraction of the tolerance, so that the
+// FP64 residual that follows usually meets it
+#ifndef PCG_FP32_INNER
+#define PCG_FP32_INNER 1.0
+#endif
 #ifndef PCG_CHUNK
 #define PCG_CHUNK 8
 #endif
@@ -1568,7 +1573,7 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
             k_pcg_upd1<V, M><<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->dmask.p, Minv, p, q, c->qc.p, c->sol.p, r, z,
                                                    c->part_c.p, nsp, c->part_a.p, c->part_b.p, c->pcg_state.p, it);
             k_pcg_upd2<V><<<nu, TPB, 0, c->s>>>(nv, c->scal.p, z, p, c->part_a.p, c->part_b.p, nu, c->pcg_state.p,
-                                                it, tol2);
+                                                it, prec == 1 ? tol2 * (PCG_FP32_INNER * PCG_FP32_INNER) : tol2);
             STCHK(fipc_tm_end(c, TM_PCGVEC));
         }
         HIPCHK(hipGetLastError());
