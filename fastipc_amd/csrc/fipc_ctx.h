@@ -139,6 +139,11 @@ struct fipc_ctx {
     DVec<double> part_a, part_b, part_c, scal;
     DVec<int> x0flag;      // -1 when the PCG start x0 (Dirichlet increments) is zero
     hipEvent_t ws_done = nullptr;  // end of the last warm-start prediction on this stream
+    // second stream: the contact assembly, overlapped with the tet pass
+    // (ev_rhs: inertia term and cleared diagonal blocks ready; ev_contact:
+    // contact rows assembled)
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_rhs = nullptr, ev_contact = nullptr;
     // elastic and barrier energy of the last evaluation, valid for its x
     // version and constraint-set version (cons_version: bumped by every
     // constraint search); parameter changes clear e_ok
@@ -228,6 +233,6 @@ static inline int d2h_wait(fipc_ctx* c)
     return FIPC_OK;
 }
 
-int fipc_tm_begin(fipc_ctx* c, int which);
-int fipc_tm_end(fipc_ctx* c, int which);
+int fipc_tm_begin(fipc_ctx* c, int which, hipStream_t st = nullptr);
+int fipc_tm_end(fipc_ctx* c, int which, hipStream_t st = nullptr);
 int fipc_tm_fold(fipc_ctx* c);

@@ -252,21 +252,20 @@ __global__ void __launch_bounds__(TPB) k_gather_blocks(int nnzb, const int* __re
     if (valf) valf[(size_t)b * 9 + e] = (float)acc;
 }
 
-// the gradient: inertia term plus the vertex's tet contributions in order
-// (rhs = -g; rhs[D] = 0, NEWTON_IPC.py:772), and the contact diagonal blocks
-// cleared for k_contact.  Three lanes per vertex, one per component, 21
+// the gradient in two parts around the contact assembly, which runs on a
+// second stream meanwhile: k_rhs_init writes the inertia term
+// (compute_inertia, NEWTON_IPC.py:558-567; rhs = -g, rhs[D] = 0,
+// NEWTON_IPC.py:772) and clears the contact diagonal blocks before
+// k_contact adds its rows; k_gather_tetg then subtracts the vertex's tet
+// contributions in order.  Three lanes per vertex, one per component, 21
 // vertices per wavefront (lane 63 idles): the lanes of a vertex read each
-// 24-byte contribution together.  Each component is summed in the order of a
-// single thread.
+// 24-byte contribution together.
 constexpr int RHS_VPW = 21;                     // vertices per wavefront
 constexpr int RHS_VPG = RHS_VPW * (TPB / 64);   // vertices per workgroup
 static inline int rhs_grid(long nv) { return (int)((nv + RHS_VPG - 1) / RHS_VPG); }
-__global__ void __launch_bounds__(TPB) k_gather_rhs(int nv, const int* __restrict__ vcptr,
-                                                    const int* __restrict__ vcent, const double* __restrict__ tetg,
-                                                    const double* __restrict__ m, const double* __restrict__ x,
-                                                    const double* __restrict__ xt,
-                                                    const unsigned char* __restrict__ dmask, double* rhs,
-                                                    double* cdiag)
+__global__ void __launch_bounds__(TPB) k_rhs_init(int nv, const double* __restrict__ m, const double* __restrict__ x,
+                                                  const double* __restrict__ xt,
+                                                  const unsigned char* __restrict__ dmask, double* rhs, double* cdiag)
 {
     const int lane = threadIdx.x & 63;
     if (lane >= 3 * RHS_VPW) return;
@@ -274,11 +273,18 @@ __global__ void __launch_bounds__(TPB) k_gather_rhs(int nv, const int* __restric
     const int d = lane % 3;
     if (i >= nv) return;
     for (int k = 0; k < 3; k++) cdiag[9 * i + 3 * d + k] = 0.0;
-    if (dmask[i]) {
-        rhs[3 * i + d] = 0.0;
-        return;
-    }
-    double r = -m[i] * (x[3 * i + d] - xt[3 * i + d]);
+    rhs[3 * i + d] = dmask[i] ? 0.0 : -m[i] * (x[3 * i + d] - xt[3 * i + d]);
+}
+__global__ void __launch_bounds__(TPB) k_gather_tetg(int nv, const int* __restrict__ vcptr,
+                                                     const int* __restrict__ vcent, const double* __restrict__ tetg,
+                                                     const unsigned char* __restrict__ dmask, double* rhs)
+{
+    const int lane = threadIdx.x & 63;
+    if (lane >= 3 * RHS_VPW) return;
+    const long i = ((long)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)) * RHS_VPW + lane / 3;
+    const int d = lane % 3;
+    if (i >= nv || dmask[i]) return;
+    double r = rhs[3 * i + d];
     const int t1 = vcptr[i + 1];
 #pragma unroll 4
     for (int t = vcptr[i]; t < t1; t++) r -= tetg[(size_t)vcent[t] * 3 + d];
@@ -1047,19 +1053,22 @@ static hipEvent_t pool_get(fipc_ctx* c)
     c->event_pool.pop_back();
     return e;
 }
-int fipc_tm_begin(fipc_ctx* c, int which)
+// the events of a timed class are recorded on the stream its launches use
+// (c->s unless given); fipc_tm_fold waits for c->s, which waits for the
+// second stream's work before using it
+int fipc_tm_begin(fipc_ctx* c, int which, hipStream_t st)
 {
     if (!c->timing) return FIPC_OK;
     hipEvent_t a = pool_get(c), b = pool_get(c);
     if (!a || !b) return FIPC_ERR_HIP;
-    HIPCHK(hipEventRecord(a, c->s));
+    HIPCHK(hipEventRecord(a, st ? st : c->s));
     c->tm[which].pending.push_back({a, b});
     return FIPC_OK;
 }
-int fipc_tm_end(fipc_ctx* c, int which)
+int fipc_tm_end(fipc_ctx* c, int which, hipStream_t st)
 {
     if (!c->timing) return FIPC_OK;
-    HIPCHK(hipEventRecord(c->tm[which].pending.back().second, c->s));
+    HIPCHK(hipEventRecord(c->tm[which].pending.back().second, st ? st : c->s));
     if (c->tm[which].pending.size() > 4096) STCHK(fipc_tm_fold(c));
     return FIPC_OK;
 }
@@ -1096,6 +1105,9 @@ extern "C" int fipc_create(fipc_ctx** out)
     if (ndev <= 0) return FIPC_ERR_HIP;
     fipc_ctx* c = new fipc_ctx();
     HIPCHK(hipStreamCreate(&c->s));
+    HIPCHK(hipStreamCreate(&c->s2));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_rhs, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_contact, hipEventDisableTiming));
     HIPCHK(hipHostMalloc((void**)&c->pin, FIPC_PIN_BYTES, hipHostMallocDefault));
     *out = c;
     return FIPC_OK;
@@ -1105,7 +1117,11 @@ extern "C" int fipc_destroy(fipc_ctx* c)
 {
     if (!c) return FIPC_OK;
     (void)hipStreamSynchronize(c->s);
+    if (c->s2) (void)hipStreamSynchronize(c->s2);
     if (c->pin) (void)hipHostFree(c->pin);
+    if (c->ev_rhs) (void)hipEventDestroy(c->ev_rhs);
+    if (c->ev_contact) (void)hipEventDestroy(c->ev_contact);
+    if (c->s2) (void)hipStreamDestroy(c->s2);
     if (c->ws_done) (void)hipEventDestroy(c->ws_done);
     release(c->tets); release(c->bp); release(c->be); release(c->bt); release(c->dids);
     release(c->IB); release(c->vol); release(c->la); release(c->mu); release(c->m); release(c->dmask);
@@ -1417,7 +1433,7 @@ extern "C" int fipc_residual(fipc_ctx* c, double* res)
 }
 
 // ---- assembly ----
-static int launch_contacts(fipc_ctx* c, int accumulate)
+static int launch_contacts(fipc_ctx* c, int accumulate, hipStream_t st)
 {
     ContactBatch cb;
     for (int t = 0; t < NTYPES; t++) {
@@ -1427,8 +1443,8 @@ static int launch_contacts(fipc_ctx* c, int accumulate)
     }
     int nb = contact_blocks(cb);
     if (!nb) return FIPC_OK;
-    k_contact<<<nb, TPB, 0, c->s>>>(cb, c->x.p, c->x0.p, c->dHat2, c->kappa, c->rhs.p, c->cdiag.p, nullptr, nullptr, 1,
-                                    accumulate, c->dmask.p, 20);
+    k_contact<<<nb, TPB, 0, st>>>(cb, c->x.p, c->x0.p, c->dHat2, c->kappa, c->rhs.p, c->cdiag.p, nullptr, nullptr, 1,
+                                  accumulate, c->dmask.p, 20);
     HIPCHK(hipGetLastError());
     return FIPC_OK;
 }
@@ -1438,6 +1454,20 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     if (!c || !c->nv) return FIPC_ERR_STATE;
     c->precision = precision;
     c->sys_version++;
+    if (precision == 1) STCHK(ensure(c->valf, (size_t)c->nnzb * 9));
+    if (precision == 1) STCHK(ensure(c->Minvf, (size_t)c->nv * 9));
+    // the contact rows (few, latency-bound) run on the second stream while
+    // the tet pass and the block gather fill the GPU: the second stream
+    // starts after the inertia term and the cleared diagonal blocks, and the
+    // first waits for it before the tet gradients and the preconditioner
+    k_rhs_init<<<rhs_grid(c->nv), TPB, 0, c->s>>>(c->nv, c->m.p, c->x.p, c->xt.p, c->dmask.p, c->rhs.p, c->cdiag.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev_rhs, c->s));
+    HIPCHK(hipStreamWaitEvent(c->s2, c->ev_rhs, 0));
+    STCHK(fipc_tm_begin(c, TM_CONTACT, c->s2));
+    STCHK(launch_contacts(c, 1, c->s2));
+    STCHK(fipc_tm_end(c, TM_CONTACT, c->s2));
+    HIPCHK(hipEventRecord(c->ev_contact, c->s2));
     STCHK(fipc_tm_begin(c, TM_ELASTIC));
     if (c->nt)
         k_elastic<<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p, c->x.p,
@@ -1445,17 +1475,13 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_ELASTIC));
     STCHK(fipc_tm_begin(c, TM_GATHER));
-    if (precision == 1) STCHK(ensure(c->valf, (size_t)c->nnzb * 9));
     k_gather_blocks<<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->tetH.p, c->brow.p, c->col.p, c->m.p,
                                                      c->val.p, precision == 1 ? c->valf.p : nullptr);
-    k_gather_rhs<<<rhs_grid(c->nv), TPB, 0, c->s>>>(c->nv, c->vcptr.p, c->vcent.p, c->tetg.p, c->m.p, c->x.p, c->xt.p,
-                                                 c->dmask.p, c->rhs.p, c->cdiag.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamWaitEvent(c->s, c->ev_contact, 0));
+    k_gather_tetg<<<rhs_grid(c->nv), TPB, 0, c->s>>>(c->nv, c->vcptr.p, c->vcent.p, c->tetg.p, c->dmask.p, c->rhs.p);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_GATHER));
-    STCHK(fipc_tm_begin(c, TM_CONTACT));
-    STCHK(launch_contacts(c, 1));
-    STCHK(fipc_tm_end(c, TM_CONTACT));
-    if (precision == 1) STCHK(ensure(c->Minvf, (size_t)c->nv * 9));
     k_precond<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->val.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p,
                                              precision == 1 ? c->Minvf.p : nullptr);
     HIPCHK(hipGetLastError());
