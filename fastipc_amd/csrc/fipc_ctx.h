@@ -144,6 +144,10 @@ struct fipc_ctx {
     // contact rows assembled)
     hipStream_t s2 = nullptr;
     hipEvent_t ev_rhs = nullptr, ev_contact = nullptr;
+    // max |sol| read back at the end of a solve; sol_version counts the
+    // writes of sol, res_version is the version res_cached belongs to
+    double res_cached = 0.0;
+    unsigned long long sol_version = 1, res_version = 0;
     // elastic and barrier energy of the last evaluation, valid for its x
     // version and constraint-set version (cons_version: bumped by every
     // constraint search); parameter changes clear e_ok

@@ -1273,11 +1273,12 @@ extern "C" int fipc_upload_mesh(fipc_ctx* c, int nv, const double* x0, const dou
     HIPCHK(hipMemsetAsync(c->dprev.p, 0, n3 * 8, c->s));
     c->nfree = -1;
     HIPCHK(hipMemsetAsync(c->sol.p, 0, n3 * 8, c->s));
+    c->sol_version++;
     HIPCHK(hipMemsetAsync(c->rhs.p, 0, n3 * 8, c->s));
     int nmax = std::max(std::max(nv, nt), 4096);
-    // block partials: k_pcg_init writes one per block of nblk(nv); the SpMV
-    // grid has at most SPMV_MAXB blocks
-    STCHK(ensure(c->part_a, nblk(nmax) + 64)); STCHK(ensure(c->part_b, nblk(nmax) + 64));
+    // block partials: k_pcg_init writes one per block of nblk(nv), k_maxabs
+    // (into part_a) at most 1024; the SpMV grid has at most SPMV_MAXB blocks
+    STCHK(ensure(c->part_a, std::max(nblk(nmax), 1024) + 64)); STCHK(ensure(c->part_b, nblk(nmax) + 64));
     STCHK(ensure(c->part_c, SPMV_MAXB + 64));
     STCHK(ensure(c->scal, 8)); STCHK(ensure(c->pcg_state, 4));
     STCHK(ensure(c->epart, nblk(nmax) * 4 + 64));
@@ -1358,6 +1359,7 @@ extern "C" int fipc_set_vector(fipc_ctx* c, int which, const double* in)
     if (d == &c->x) c->x_version++;
     if (d != &c->sol) c->sys_version++;   // a continued solve needs the system it solved
     c->restart_ready = false;
+    if (d == &c->sol) c->sol_version++;
     HIPCHK(hipStreamSynchronize(c->s));
     return FIPC_OK;
 }
@@ -1420,6 +1422,10 @@ extern "C" int fipc_apply_sol(fipc_ctx* c, double alpha)
 extern "C" int fipc_residual(fipc_ctx* c, double* res)
 {
     if (!c || !c->nv || !res) return FIPC_ERR_STATE;
+    if (c->res_version == c->sol_version) {  // read back at the end of the solve that wrote sol
+        *res = c->res_cached;
+        return FIPC_OK;
+    }
     int nb = std::min(nblk(c->nv * 3), 1024);
     k_maxabs<<<nb, TPB, 0, c->s>>>(c->nv * 3, c->sol.p, c->part_a.p);
     HIPCHK(hipGetLastError());
@@ -1641,6 +1647,7 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
     // solve it continues, when there is one (its FP64 residual is current)
     bool reuse = first_round > 0 && prec == 1 && c->restart_ready;
     c->restart_ready = false;
+    c->sol_version++;  // this solve writes sol
     for (int round = first_round; round < first_round + rounds; round++) {
         if (round == 0) {
             // x0 = prescribed Dirichlet increments
@@ -1677,9 +1684,21 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
     // a solve that ended on a restart whose FP64 residual met the tolerance
     // leaves that restart for a continuation
     c->restart_ready = prec == 1 && init_only;
+    // the scalars and max |sol| (the Newton residual, output_residual) are
+    // read back together: fipc_residual returns the kept value while sol is
+    // unchanged
     double sc[5];
+    const int nbm = std::min(nblk(nv * 3), 1024);
+    k_maxabs<<<nbm, TPB, 0, c->s>>>(nv * 3, c->sol.p, c->part_a.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> hm(nbm);
     STCHK(d2h_async(c, sc, c->scal.p, 40));
+    STCHK(d2h_async(c, hm.data(), c->part_a.p, (size_t)nbm * 8));
     STCHK(d2h_wait(c));
+    double mx = 0.0;
+    for (double v : hm) mx = std::max(mx, v);
+    c->res_cached = mx;
+    c->res_version = c->sol_version;
     double rel = sc[2] > 0 ? sqrt(sc[4] / sc[2]) : 0.0;
     c->last_pcg_iters = total;
     c->last_pcg_relres = rel;

@@ -225,6 +225,7 @@ extern "C" int fipc_ws_predict(fipc_ctx* c, fipc_mlp* m, double scale)
     STCHK(mlp_forward_dev(m, c->s, c->nv, c->wsX.p, &Y));
     k_ws_apply<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, Y, 3, scale, c->x.p, c->xt.p, c->dprev.p, c->dmask.p, c->sol.p);
     c->restart_ready = false;  // sol changed
+    c->sol_version++;
     HIPCHK(hipGetLastError());
     // no host wait: later work on the network's own stream (training,
     // parameter changes, destruction) waits for this stream's use of its
