@@ -17,8 +17,9 @@ Workload: the compressed slab, a 50 x 80 x 50-vertex lattice (200,000
 vertices, 948,395 tets, E = 1e6, nu = 0.45) between two Dirichlet plates that
 close at 0.2 slab heights per second with a 30 degree sinusoidal twist,
 dt = 0.005, dHat = 1e-3 bbox diagonal, Newton tolerance 1e-2 dt bbox.  W
-warm-up steps (the plates reach the slab) run untimed, then K steps are timed,
-each ending with a device synchronisation.  The position state is resident in
+warm-up steps (the plates reach the slab) run untimed, then K steps are timed
+between two device synchronisations (every step also waits for its own
+results on the host: the line search, the Newton test and the collision check).  The position state is resident in
 HBM for the whole run.
 
 Printed: one JSON line with steps/s, the roofline figures of the dominant

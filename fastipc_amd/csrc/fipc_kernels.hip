@@ -1402,8 +1402,7 @@ extern "C" int fipc_end_step(fipc_ctx* c)
     k_end_step<<<nblk(c->nv * 3), TPB, 0, c->s>>>(c->nv * 3, c->x.p, c->xn.p, c->xt.p, c->dmask.p, c->v.p, c->dprev.p,
                                                    1.0 / c->dt);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->s));
-    return FIPC_OK;
+    return FIPC_OK;  // ordered before the next step's work; host reads of v synchronise
 }
 extern "C" int fipc_save_prev(fipc_ctx* c)
 {
