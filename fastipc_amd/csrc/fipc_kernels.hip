@@ -597,6 +597,10 @@ __global__ void k_precond(int nv, const double* __restrict__ val, const int* __r
 // BSR rows, p_v . (H_r p_r)_v for contact lanes (Dirichlet entries of the
 // PCG directions are zero, so this is the masked p.q).
 #define SPMV_ROWS (TPB / SPMV_G)
+// entries of a row per lane whose loads are in flight together
+#ifndef SPMV_UNROLL
+#define SPMV_UNROLL _Pragma("unroll 2")
+#endif
 #ifndef SPMV_MAXB
 #define SPMV_MAXB 1024
 #endif
@@ -686,7 +690,7 @@ __global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ ro
             // the row's stored blocks (i, j), j >= i, then the transposes of
             // the blocks (j, i), j < i, stored in the rows above
             double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-#pragma unroll 2
+            SPMV_UNROLL
             for (int k = lane; k < n; k += SPMV_G) {
                 bool tr = k >= nu;
                 int b, c;
