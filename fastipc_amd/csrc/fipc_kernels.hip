@@ -1108,11 +1108,15 @@ extern "C" int fipc_create(fipc_ctx** out)
     HIPCHK(hipGetDeviceCount(&ndev));
     if (ndev <= 0) return FIPC_ERR_HIP;
     fipc_ctx* c = new fipc_ctx();
-    HIPCHK(hipStreamCreate(&c->s));
-    HIPCHK(hipStreamCreate(&c->s2));
-    HIPCHK(hipEventCreateWithFlags(&c->ev_rhs, hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&c->ev_contact, hipEventDisableTiming));
-    HIPCHK(hipHostMalloc((void**)&c->pin, FIPC_PIN_BYTES, hipHostMallocDefault));
+    hipError_t e = hipStreamCreate(&c->s);
+    if (e == hipSuccess) e = hipStreamCreate(&c->s2);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_rhs, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_contact, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->pin, FIPC_PIN_BYTES, hipHostMallocDefault);
+    if (e != hipSuccess) {  // release what was created
+        (void)fipc_destroy(c);
+        HIPCHK(e);
+    }
     *out = c;
     return FIPC_OK;
 }
@@ -1120,7 +1124,7 @@ extern "C" int fipc_create(fipc_ctx** out)
 extern "C" int fipc_destroy(fipc_ctx* c)
 {
     if (!c) return FIPC_OK;
-    (void)hipStreamSynchronize(c->s);
+    if (c->s) (void)hipStreamSynchronize(c->s);
     if (c->s2) (void)hipStreamSynchronize(c->s2);
     if (c->pin) (void)hipHostFree(c->pin);
     if (c->ev_rhs) (void)hipEventDestroy(c->ev_rhs);
@@ -1150,9 +1154,9 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->ccd_pairs[0]); release(c->ccd_pairs[1]);
     release(c->dprev); release(c->wsX); release(c->ckey); release(c->free_ids); release(c->cap_ids);
     release(c->cap_X); release(c->cap_T); release(c->near_ids); release(c->nnear);
-    (void)fipc_tm_fold(c);
+    if (c->s) (void)fipc_tm_fold(c);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(c->s);
+    if (c->s) (void)hipStreamDestroy(c->s);
     delete c;
     return FIPC_OK;
 }
