@@ -1046,11 +1046,14 @@ __global__ void k_batch_tet(int n, const double* Fin, const double* IB, const do
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// timing events: no system-scope fence (they only time the kernels between
+// them; host reads of results go through stream synchronisation), which
+// keeps their own cost out of the bracketed launch
 static hipEvent_t pool_get(fipc_ctx* c)
 {
     if (c->event_pool.empty()) {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
         return e;
     }
     hipEvent_t e = c->event_pool.back();
