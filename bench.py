@@ -186,8 +186,12 @@ def run_gpu(args, variant):
                               "launches": kt["elastic_tet_pass"]["launches"],
                               "us_avg": kt["elastic_tet_pass"]["us_avg"] + kt["assembly_gather"]["us_avg"]}
     nbytes = algorithmic_bytes(sim, precision)
-    # dominant kernel with a byte model
-    dom = max(nbytes, key=lambda k: kt[k]["ms_total"])
+    # dominant kernel with a byte model: the single kernel with the most time
+    # (the SpMV is one kernel; the assembly class is the tet pass plus two
+    # gathers, so it competes through its largest kernel, the tet pass)
+    single = {"spmv": kt["spmv"]["ms_total"],
+              "elastic_assembly": max(kt["elastic_tet_pass"]["ms_total"], kt["assembly_gather"]["ms_total"])}
+    dom = max(nbytes, key=lambda k: single[k])
     avg_s = kt[dom]["us_avg"] * 1e-6
     achieved = nbytes[dom] / avg_s / 1e9 if avg_s > 0 else 0.0
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
