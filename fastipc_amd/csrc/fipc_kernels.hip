@@ -1061,8 +1061,7 @@ static hipEvent_t pool_get(fipc_ctx* c)
     return e;
 }
 // the events of a timed class are recorded on the stream its launches use
-// (c->s unless given); fipc_tm_fold waits for c->s, which waits for the
-// second stream's work before using it
+// (c->s unless given); fipc_tm_fold waits for both streams
 int fipc_tm_begin(fipc_ctx* c, int which, hipStream_t st)
 {
     if (!c->timing) return FIPC_OK;
@@ -1082,6 +1081,7 @@ int fipc_tm_end(fipc_ctx* c, int which, hipStream_t st)
 int fipc_tm_fold(fipc_ctx* c)
 {
     HIPCHK(hipStreamSynchronize(c->s));
+    if (c->s2) HIPCHK(hipStreamSynchronize(c->s2));  // its timed launches too
     for (auto& t : c->tm) {
         for (auto& pr : t.pending) {
             float ms = 0.f;
