@@ -1,7 +1,7 @@
 """Benchmark: simulation steps/s of the IPC Newton step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode VARIANT] [--all-variants]
-                    [--pcg-tol 1e-6] [--pcg-tol-search 1e-2]
+                    [--pcg-tol 1e-6] [--pcg-tol-search 2e-2]
 
 The value is the steps/s of --mode (default "chosen+inexact": the learned
 Newton warm start with the FP32 Hessian and FP64 residual refinement, search
@@ -282,7 +282,7 @@ def main():
                     help="skip the plain FP64 runs (exact and inexact directions) timed for the ratios")
     ap.add_argument("--all-variants", action="store_true", help="also time every other variant")
     ap.add_argument("--pcg-tol", type=float, default=1e-6)
-    ap.add_argument("--pcg-tol-search", type=float, default=1e-2,
+    ap.add_argument("--pcg-tol-search", type=float, default=2e-2,
                     help="PCG tolerance of the search directions of the +inexact variants")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
