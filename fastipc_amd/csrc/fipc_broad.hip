@@ -207,26 +207,30 @@ __global__ void __launch_bounds__(TPB) k_grid_query(int nq, const int* __restric
     }
 }
 
-static int scan_inplace(fipc_ctx* c, int* d, int n)
+// st: the stream (c->s if null); each stream has its own scan scratch
+static int scan_inplace(fipc_ctx* c, int* d, int n, hipStream_t st = nullptr)
 {
+    const hipStream_t s = st ? st : c->s;
+    DVec<unsigned char>& tmp = s == c->s ? c->cub_tmp : c->cub_tmp2;
     size_t bytes = 0;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, d, d, n, c->s));
-    if (bytes > c->cub_tmp.cap) STCHK(ensure(c->cub_tmp, bytes));
-    bytes = c->cub_tmp.cap;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, bytes, d, d, n, c->s));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, d, d, n, s));
+    if (bytes > tmp.cap) STCHK(ensure(tmp, bytes));
+    bytes = tmp.cap;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, d, d, n, s));
     return FIPC_OK;
 }
 
 // Build the grid over n items without waiting for the host: buffers are
 // sized by the capacity, overflow is flagged in flag[0] (see GF_RETRY).
-static int grid_build(fipc_ctx* c, int slot, int n, const BoxFn& box)
+static int grid_build(fipc_ctx* c, int slot, int n, const BoxFn& box, hipStream_t st = nullptr)
 {
+    const hipStream_t s = st ? st : c->s;
     Grid& g = c->grid[slot];
     // the same boxes at the same x: the grid is already there
     if (g.reusable && !box.sol && g.x_version == c->x_version && g.kind == box.kind && g.n == n && g.infl == box.infl)
         return FIPC_OK;
     g.reusable = false;
-    STCHK(fipc_tm_begin(c, TM_GRID));
+    STCHK(fipc_tm_begin(c, TM_GRID, s));
     int cap = g.cap;
     // buckets: a power of two at least the entry capacity
     unsigned M = 1024;
@@ -236,17 +240,17 @@ static int grid_build(fipc_ctx* c, int slot, int n, const BoxFn& box)
     STCHK(ensure(g.ekey, cap)); STCHK(ensure(g.eitem, cap)); STCHK(ensure(g.items, cap));
     STCHK(ensure(g.ecell, cap)); STCHK(ensure(g.cells, cap));
     STCHK(ensure(g.bstart, (size_t)M + 1));
-    k_grid_count<<<nblk(n + 1), TPB, 0, c->s>>>(n, box, c->inv_dx, g.cnt.p, c->flag.p);
+    k_grid_count<<<nblk(n + 1), TPB, 0, s>>>(n, box, c->inv_dx, g.cnt.p, c->flag.p);
     HIPCHK(hipGetLastError());
-    STCHK(scan_inplace(c, g.cnt.p, n + 1));
+    STCHK(scan_inplace(c, g.cnt.p, n + 1, s));
     const int* nep = g.cnt.p + n;
     int nb = stride_grid(cap);
-    HIPCHK(hipMemsetAsync(g.bstart.p, 0, ((size_t)M + 1) * 4, c->s));
-    k_grid_fill<<<nb, TPB, 0, c->s>>>(n, cap, box, c->inv_dx, M, g.cnt.p, g.ekey.p, g.eitem.p, g.ecell.p, g.bstart.p,
-                                      c->flag.p);
+    HIPCHK(hipMemsetAsync(g.bstart.p, 0, ((size_t)M + 1) * 4, s));
+    k_grid_fill<<<nb, TPB, 0, s>>>(n, cap, box, c->inv_dx, M, g.cnt.p, g.ekey.p, g.eitem.p, g.ecell.p, g.bstart.p,
+                                   c->flag.p);
     HIPCHK(hipGetLastError());
-    STCHK(scan_inplace(c, g.bstart.p, M + 1));
-    k_grid_scatter<<<nb, TPB, 0, c->s>>>(nep, cap, g.ekey.p, g.eitem.p, g.ecell.p, g.bstart.p, g.items.p, g.cells.p);
+    STCHK(scan_inplace(c, g.bstart.p, M + 1, s));
+    k_grid_scatter<<<nb, TPB, 0, s>>>(nep, cap, g.ekey.p, g.eitem.p, g.ecell.p, g.bstart.p, g.items.p, g.cells.p);
     HIPCHK(hipGetLastError());
     // reusable once built; an overflow found after the operation's
     // synchronisation clears this again (bp_check)
@@ -255,25 +259,27 @@ static int grid_build(fipc_ctx* c, int slot, int n, const BoxFn& box)
     g.kind = box.kind;
     g.n = n;
     g.infl = box.infl;
-    STCHK(fipc_tm_end(c, TM_GRID));
+    STCHK(fipc_tm_end(c, TM_GRID, s));
     return FIPC_OK;
 }
 
 // query the last grid with nq boxes; no host synchronisation
 template <class Visit>
-static int run_query(fipc_ctx* c, int slot, int nq, const BoxFn& qbox, Visit visit, int timer)
+static int run_query(fipc_ctx* c, int slot, int nq, const BoxFn& qbox, Visit visit, int timer,
+                     hipStream_t st = nullptr)
 {
     if (nq == 0) return FIPC_OK;
+    const hipStream_t s = st ? st : c->s;
     Grid& g = c->grid[slot];
     STCHK(ensure(g.qoff, (size_t)nq + 2));
-    STCHK(fipc_tm_begin(c, timer));
-    k_query_cells<<<nblk(nq + 1), TPB, 0, c->s>>>(nq, qbox, c->inv_dx, g.qoff.p, c->flag.p);
+    STCHK(fipc_tm_begin(c, timer, s));
+    k_query_cells<<<nblk(nq + 1), TPB, 0, s>>>(nq, qbox, c->inv_dx, g.qoff.p, c->flag.p);
     HIPCHK(hipGetLastError());
-    STCHK(scan_inplace(c, g.qoff.p, nq + 1));
-    k_grid_query<<<GRID_MAXB, TPB, 0, c->s>>>(nq, g.qoff.p, qbox, c->inv_dx, g.M, g.bstart.p, g.items.p, g.cells.p,
-                                                c->flag.p, visit);
+    STCHK(scan_inplace(c, g.qoff.p, nq + 1, s));
+    k_grid_query<<<GRID_MAXB, TPB, 0, s>>>(nq, g.qoff.p, qbox, c->inv_dx, g.M, g.bstart.p, g.items.p, g.cells.p,
+                                             c->flag.p, visit);
     HIPCHK(hipGetLastError());
-    STCHK(fipc_tm_end(c, timer));
+    STCHK(fipc_tm_end(c, timer, s));
     return FIPC_OK;
 }
 
@@ -661,6 +667,23 @@ extern "C" int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* al
         unsigned long long init[3];
         for (int i = 0; i < 3; i++) memcpy(&init[i], &one, 8);
         HIPCHK(hipMemcpyAsync(c->amin.p, init, 24, hipMemcpyHostToDevice, c->s));
+        // the edge-edge half (the longer one) runs on the second stream while
+        // the point-triangle half and the inversion bound run on the first:
+        // separate grids, pair lists, counters and minima; the first waits
+        // for the second before the readback
+        HIPCHK(hipEventRecord(c->ev_rhs, c->s));
+        HIPCHK(hipStreamWaitEvent(c->s2, c->ev_rhs, 0));
+        BoxFn edges{c->x.p, c->sol.p, c->be.p, 1, 0.0};
+        STCHK(grid_build(c, 1, c->nbe, edges, c->s2));
+        BoxFn qedges{c->x.p, c->sol.p, c->be.p, 1, c->dHat};
+        CandCCDEE vee{c->x.p, c->sol.p, c->be.p, c->dHat, eta, PairSink{pc + 1, cap, c->ccd_pairs[1].p}};
+        STCHK(run_query(c, 1, c->nbe, qedges, vee, TM_CCD, c->s2));
+        STCHK(fipc_tm_begin(c, TM_CCD, c->s2));
+        k_ccd_ee<<<stride_grid(cap), TPB, 0, c->s2>>>(pc + 1, cap, c->ccd_pairs[1].p, c->x.p, c->sol.p, c->be.p, eta,
+                                                      c->amin.p + 1);
+        HIPCHK(hipGetLastError());
+        STCHK(fipc_tm_end(c, TM_CCD, c->s2));
+        HIPCHK(hipEventRecord(c->ev_contact, c->s2));
         BoxFn pts{c->x.p, c->sol.p, c->bp.p, 0, c->dHat};
         STCHK(grid_build(c, 0, c->nbp, pts));
         BoxFn tris{c->x.p, c->sol.p, c->bt.p, 2, 0.0};
@@ -671,17 +694,8 @@ extern "C" int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* al
                                                      c->amin.p);
         HIPCHK(hipGetLastError());
         STCHK(fipc_tm_end(c, TM_CCD));
-        BoxFn edges{c->x.p, c->sol.p, c->be.p, 1, 0.0};
-        STCHK(grid_build(c, 1, c->nbe, edges));
-        BoxFn qedges{c->x.p, c->sol.p, c->be.p, 1, c->dHat};
-        CandCCDEE vee{c->x.p, c->sol.p, c->be.p, c->dHat, eta, PairSink{pc + 1, cap, c->ccd_pairs[1].p}};
-        STCHK(run_query(c, 1, c->nbe, qedges, vee, TM_CCD));
-        STCHK(fipc_tm_begin(c, TM_CCD));
-        k_ccd_ee<<<stride_grid(cap), TPB, 0, c->s>>>(pc + 1, cap, c->ccd_pairs[1].p, c->x.p, c->sol.p, c->be.p, eta,
-                                                     c->amin.p + 1);
-        HIPCHK(hipGetLastError());
-        STCHK(fipc_tm_end(c, TM_CCD));
         if (inversion) STCHK(fipc_inversion_alpha(c, c->amin.p + 2));
+        HIPCHK(hipStreamWaitEvent(c->s, c->ev_contact, 0));
         unsigned long long out[3];
         int hflag[3], hc[2];
         STCHK(d2h_async(c, out, c->amin.p, 24));

@@ -125,7 +125,7 @@ struct fipc_ctx {
     DVec<double> cH[NTYPES];
     DVec<unsigned long long> keys_pp, keys_pe, keys_tmp;
     DVec<int> counters;    // NTYPES raw append counters + misc
-    DVec<unsigned char> cub_tmp;
+    DVec<unsigned char> cub_tmp, cub_tmp2;  // scan / sort scratch of the first and second stream
     size_t cub_bytes = 0;
     DVec<int> nuniq;
     // preconditioner + PCG

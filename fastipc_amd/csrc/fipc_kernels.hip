@@ -1143,7 +1143,7 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->lptr); release(c->lent);
     release(c->vcent);
     for (int t = 0; t < NTYPES; t++) { release(c->rows[t]); release(c->cH[t]); }
-    release(c->keys_pp); release(c->keys_pe); release(c->keys_tmp); release(c->counters); release(c->cub_tmp);
+    release(c->keys_pp); release(c->keys_pe); release(c->keys_tmp); release(c->counters); release(c->cub_tmp); release(c->cub_tmp2);
     release(c->nuniq); release(c->cdiag); release(c->Minv); release(c->r); release(c->z); release(c->pp); release(c->q); release(c->qc);
     release(c->Minvf); release(c->rf); release(c->zf); release(c->pf); release(c->qf);
     release(c->trow); release(c->tval);
