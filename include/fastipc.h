@@ -13,7 +13,8 @@
  *     function of projects/FastIPC/NEWTON_IPC.py.  All return 0 on success or
  *     a FIPC_ERR_* status.  A context owns its device buffers; calls on one
  *     context must come from one host thread and are ordered on the
- *     context's HIP stream (fipc_assemble also runs the contact rows on a
+ *     context's HIP stream (fipc_assemble and fipc_ccd_alpha also run part
+ *     of their work -- the contact rows, the edge-edge CCD half -- on a
  *     second stream of the context, which the first waits for before the
  *     call's later work).  Host arrays are copied in and out.
  *
