@@ -185,6 +185,46 @@ def test_inversion_certificate_is_sound():
     assert cleared > 300 and kept > 300, (cleared, kept)
 
 
+def test_inversion_bound_value():
+    """The bound's value: 0.8 times the smallest positive real root of the
+    cubic t -> det[D_s + t dD_s] (coefficients fitted by _det_cubic from four
+    determinants, roots by numpy.roots), or 1 when there is none
+    (math_tools.py:160-181 with wrapper.cpp:220-252).  A swapped, negated or
+    dropped coefficient moves the root, so such a mistake in the oracle's
+    expansion fails here.  Tets whose roots are numerically ambiguous for the
+    1e-8 imaginary-part test (near-double roots, tiny imaginary parts) and
+    those on the reference's |a| <= 1e-8 quadratic branch are skipped."""
+    rng = np.random.default_rng(12)
+    eta = 0.2
+    checked = hits = skipped = 0
+    for trial in range(3000):
+        xs = [np.zeros(3), np.array([1.0, 0, 0]), np.array([0, 1.0, 0]), np.array([0, 0, 1.0])]
+        xs = [x + rng.normal(size=3) * 0.1 for x in xs]
+        scale = 10.0 ** rng.uniform(-1, 0.5)
+        ps = [rng.normal(size=3) * scale for _ in range(4)]
+        a, b, c, d = _det_cubic(xs, ps)
+        if abs(a) <= 1e-6:
+            skipped += 1
+            continue
+        roots = np.roots([a, b, c, d])
+        mag = max(1.0, np.abs(roots).max())
+        im = np.abs(roots.imag)
+        sep = min(abs(roots[i] - roots[j]) for i in range(3) for j in range(i + 1, 3))
+        if np.any((im > 1e-12 * mag) & (im < 1e-5 * mag)) or sep < 1e-4 * mag:
+            skipped += 1
+            continue
+        pos = [z.real for z in roots if abs(z.imag) <= 1e-12 * mag and z.real > 1e-9]
+        al = O.tet_inversion_alpha(xs, ps, eta)
+        if pos:
+            want = (1.0 - eta) * min(pos)
+            assert al == pytest.approx(want, rel=1e-9), (trial, (a, b, c, d), al, want)
+            hits += 1
+        else:
+            assert al == 1.0, (trial, (a, b, c, d), al)
+        checked += 1
+    assert checked > 2500 and hits > 500, (checked, hits, skipped)
+
+
 @pytest.mark.parametrize("name,fn", [("pt", "pt_ccd"), ("ee", "ee_ccd")])
 def test_ccd_vs_reference(name, fn):
     P, D, d2, T = (REF[f"ccd_{name}_{s}"] for s in ("pts", "disp", "d2", "toc"))
