@@ -110,7 +110,29 @@ _SCALAR = {
     "edge_edge_ccd": [_D] * 26 + [_P],
 }
 
-#: every symbol include/fastipc.h declares
+# @ti.func drop-ins (include/fastipc_tifunc.h): by-value doubles in, double*
+# out.  Per-constraint functions of common/math/ipc.py take the points (3
+# doubles each; the mollified types 4 points and their 4 rest points) then
+# dHat2 and kappa; their gradients have nd entries and Hessians nd * nd.
+for _name, _npts, _nd in (("PP", 2, 6), ("PE", 3, 9), ("PT", 4, 12), ("EE", 4, 12),
+                          ("EEM", 8, 12), ("PPM", 8, 12), ("PEM", 8, 12)):
+    _nin = 3 * _npts + 2
+    _SCALAR[f"{_name}_energy"] = [_D] * _nin + [_P]
+    _SCALAR[f"{_name}_gradient"] = [_D] * _nin + [_P] * _nd
+    _SCALAR[f"{_name}_hessian"] = [_D] * _nin + [_P] * (_nd * _nd)
+    _SCALAR[f"{_name}_g_and_H"] = [_D] * _nin + [_P] * (_nd + _nd * _nd)
+_SCALAR.update({
+    "barrier_E": [_D] * 3 + [_P], "barrier_g": [_D] * 3 + [_P], "barrier_H": [_D] * 3 + [_P],
+    "M_threshold": [_D] * 12 + [_P],
+    "PT_type": [_D] * 12 + [_P], "EE_type": [_D] * 12 + [_P], "PE_type": [_D] * 9 + [_P],
+    "PT_dist2": [_D] * 13 + [_P], "EE_dist2": [_D] * 13 + [_P],
+    "elasticity_energy": [_D] * 5 + [_P], "elasticity_gradient": [_D] * 5 + [_P] * 3,
+    "elasticity_hessian": [_D] * 5 + [_P] * 9,
+    "elasticity_first_piola_kirchoff_stress": [_D] * 11 + [_P] * 9,
+    "elasticity_first_piola_kirchoff_stress_derivative": [_D] * 11 + [_P] * 81,
+})
+
+#: every symbol include/*.h declares
 EXPORTS = tuple(_SIGS) + tuple(_SCALAR)
 
 _so = None
@@ -134,6 +156,19 @@ def load(path: str = LIB_PATH):
         f.restype = None
     _so = so
     return so
+
+
+def scalar(name, *args) -> np.ndarray:
+    """Calls a by-value / double* drop-in with the given input doubles and
+    returns its outputs as a float64 array."""
+    f = getattr(load(), name)
+    nin = sum(1 for a in f.argtypes if a is _D)
+    if len(args) != nin:
+        raise TypeError(f"{name} takes {nin} doubles, got {len(args)}")
+    out = np.zeros(len(f.argtypes) - nin)
+    base = out.ctypes.data
+    f(*[float(a) for a in args], *[C.cast(base + 8 * i, _P) for i in range(len(out))])
+    return out
 
 
 def call(name, *args):

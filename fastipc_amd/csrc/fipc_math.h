@@ -145,6 +145,13 @@ FI int EE_type(v3 a0, v3 a1, v3 b0, v3 b1)
     else if (tN >= tD) k = (-d + b <= 0.0) ? 1 : ((-d + b >= a) ? 4 : 7);
     return k;
 }
+// PE_type (distance.py:1756-1767): where p projects on the line of (e0, e1)
+FI int PE_type(v3 p, v3 e0, v3 e1)
+{
+    v3 e = e1 - e0;
+    double ratio = dot(e, p - e0) / n2(e);
+    return ratio < 0 ? 0 : (ratio > 1 ? 1 : 2);
+}
 FI double PT_dist2(v3 p, v3 t0, v3 t1, v3 t2, int c)
 {
     switch (c) {
@@ -591,6 +598,27 @@ FI void nh_P(const double* F, double la, double mu, double* P)
 //     dPdF = sum_{k,l} M[k][l] w_k w_l^T,   w_{a*3+b} = vec(u_a v_b^T)
 // M holds the projected sigma-Hessian (pairs (a,a),(b,b)) and three projected
 // 2x2 blocks coupling (a,b) with (b,a).
+// dPsi/dsigma and d2Psi/dsigma2 (neo_hookean.py:19-58), H row-major 3x3
+FI void nh_sigma_gradient(const double* s, double la, double mu, double* g)
+{
+    // derivative-only arithmetic: contraction into FMA is allowed here
+#pragma clang fp contract(fast)
+    double lp = log(s[0] * s[1] * s[2]);
+    for (int i = 0; i < 3; i++) g[i] = mu * (s[i] - 1.0 / s[i]) + la * (1.0 / s[i]) * lp;
+}
+FI void nh_sigma_hessian(const double* s, double la, double mu, double* H)
+{
+    // derivative-only arithmetic: contraction into FMA is allowed here
+#pragma clang fp contract(fast)
+    double lp = log(s[0] * s[1] * s[2]);
+    for (int i = 0; i < 3; i++) {
+        double inv2 = 1.0 / s[i] / s[i];
+        H[i * 4] = mu * (1.0 + inv2) - la * inv2 * (lp - 1.0);
+    }
+    H[1] = H[3] = la / s[0] / s[1];
+    H[5] = H[7] = la / s[1] / s[2];
+    H[2] = H[6] = la / s[0] / s[2];
+}
 struct NHFactor {
     double U[9], V[9];
     double S[3][3];   // projected d2Psi/dsigma2
@@ -604,15 +632,9 @@ FI void nh_factor(const double* F, double la, double mu, NHFactor& o)
     svd3(F, o.U, s, o.V);
     double lp = log(s[0] * s[1] * s[2]), middle = mu - la * lp;
     double g[3];
-    for (int i = 0; i < 3; i++) g[i] = mu * (s[i] - 1.0 / s[i]) + la * (1.0 / s[i]) * lp;
+    nh_sigma_gradient(s, la, mu, g);
     double H[9];
-    for (int i = 0; i < 3; i++) {
-        double inv2 = 1.0 / s[i] / s[i];
-        H[i * 4] = mu * (1.0 + inv2) - la * inv2 * (lp - 1.0);
-    }
-    H[1] = H[3] = la / s[0] / s[1];
-    H[5] = H[7] = la / s[1] / s[2];
-    H[2] = H[6] = la / s[0] / s[2];
+    nh_sigma_hessian(s, la, mu, H);
     make_pd_full<3>(H);
     for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) o.S[i][j] = H[i * 3 + j];
     const int pi[3] = {0, 1, 2}, pj[3] = {1, 2, 0};
@@ -623,6 +645,40 @@ FI void nh_factor(const double* F, double la, double mu, NHFactor& o)
         o.B[k][0] = left + right; o.B[k][1] = left - right; o.B[k][2] = left - right; o.B[k][3] = left + right;
         make_pd2(o.B[k]);
     }
+}
+
+// The full projected dP/dF (neo_hookean.py:70-131) as a 9x9 row-major matrix
+// whose rows and columns index the column-major vec of F (F[i][j] -> 3 j + i),
+// expanded from the factored form: w_(a,b)[3 j + i] = U[i][a] V[j][b].
+FI void nh_dPdF(const double* F, double la, double mu, double* D)
+{
+    NHFactor f;
+    nh_factor(F, la, mu, f);
+    double Mk[9][9];
+    for (int k = 0; k < 9; k++)
+        for (int l = 0; l < 9; l++) Mk[k][l] = 0.0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) Mk[a * 4][b * 4] = f.S[a][b];
+    const int PI[3] = {0, 1, 2}, PJ[3] = {1, 2, 0};
+    for (int k = 0; k < 3; k++) {
+        int ij = PI[k] * 3 + PJ[k], ji = PJ[k] * 3 + PI[k];
+        Mk[ij][ij] = f.B[k][0];
+        Mk[ij][ji] = f.B[k][1];
+        Mk[ji][ij] = f.B[k][2];
+        Mk[ji][ji] = f.B[k][3];
+    }
+    double w[9][9];  // w[k][vec index]
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) w[a * 3 + b][3 * j + i] = f.U[i * 3 + a] * f.V[j * 3 + b];
+    for (int r = 0; r < 9; r++)
+        for (int c = 0; c < 9; c++) {
+            double acc = 0.0;
+            for (int k = 0; k < 9; k++)
+                for (int l = 0; l < 9; l++) acc += Mk[k][l] * w[k][r] * w[l][c];
+            D[r * 9 + c] = acc;
+        }
 }
 
 // 12x12 tet Hessian as Z M Z^T (see fipc_math.h NHFactor), accumulated per

@@ -16,9 +16,23 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    src = open(os.path.join(ROOT, "include", "fastipc.h")).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|void)\s+(\w+)\s*\(", src, flags=re.M)))
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", h)).read()
+            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            names |= set(re.findall(r"^\s*(?:int|void)\s+(\w+)\s*\(", src, flags=re.M))
+    return sorted(names)
+
+
+def _decl_params(name):
+    """Parameter types of a declaration in include/*.h."""
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        m = re.search(r"^\s*(?:int|void)\s+" + name + r"\s*\(([^)]*)\)", src, flags=re.M)
+        if m:
+            return [" ".join(a.split()[:-1]) if len(a.split()) > 1 else a.strip() for a in m.group(1).split(",")]
+    raise KeyError(name)
 
 
 def test_library_exports_every_declared_symbol():
@@ -28,6 +42,23 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(so, n), n
     assert set(names) == set(_lib.EXPORTS)
+
+
+def test_tifunc_header_matches_generator():
+    """include/fastipc_tifunc.h (committed) declares exactly what
+    gen_scalar.py defines, and the ctypes signatures agree with it."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("gen_scalar", os.path.join(ROOT, "fastipc_amd", "csrc", "gen_scalar.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    assert open(os.path.join(ROOT, "include", "fastipc_tifunc.h")).read() == gen.header()
+    for name, args in _lib._SCALAR.items():
+        params = _decl_params(name)
+        want = ["double" if a is C.c_double else "double*" if a is _lib._P else "int" for a in args]
+        got = ["double*" if p.replace(" ", "") == "double*" else p.replace(" ", "") for p in params]
+        if name == "fipc_project_pd":
+            continue
+        assert got == want, name
 
 
 def test_status_without_gpu():
@@ -205,3 +236,111 @@ def test_project_pd_2_4_dropins_vs_reference(n):
         assert np.abs(got - o).max() <= 1e-12 * np.abs(a).max()
         # PSD and equal to the eigen-clamp of the leading negative run (eigh is unique for it)
         assert np.linalg.eigvalsh(got).min() >= -1e-12 * np.abs(a).max()
+
+
+# ---------------------------------------------------------------------------
+# @ti.func drop-ins (include/fastipc_tifunc.h)
+# ---------------------------------------------------------------------------
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300)
+
+
+CNAMES = ("PP", "PE", "PT", "EE", "EEM", "PPM", "PEM")
+
+
+@pytest.mark.parametrize("kind", range(7))
+def test_constraint_tifunc_dropins(kind):
+    """{T}_energy / _gradient / _hessian against the torch float64 autodiff
+    fixtures (unprojected), {T}_g_and_H against the oracle's projected
+    Hessian (ipc.py:30-329)."""
+    AD = np.load(os.path.join(ROOT, "tests", "golden", "autodiff.npz"))
+    X, X0 = AD[f"c{kind}_x"], AD[f"c{kind}_x0"]
+    h, k = float(AD["c_dhat2"]), float(AD["c_kappa"])
+    name = CNAMES[kind]
+    nd = (6, 9, 12, 12, 12, 12, 12)[kind]
+    for i in range(len(X)):
+        args = list(X[i].ravel()) + (list(X0[i].ravel()) if kind >= 4 else [])
+        E = _lib.scalar(f"{name}_energy", *args, h, k)[0]
+        g = _lib.scalar(f"{name}_gradient", *args, h, k)
+        H = _lib.scalar(f"{name}_hessian", *args, h, k).reshape(nd, nd)
+        assert E == pytest.approx(AD[f"c{kind}_E"][i], rel=1e-10)
+        assert np.abs(g - AD[f"c{kind}_g"][i]).max() <= 1e-9 * np.abs(AD[f"c{kind}_g"][i]).max()
+        assert np.abs(H - AD[f"c{kind}_H"][i]).max() <= 1e-8 * np.abs(AD[f"c{kind}_H"][i]).max()
+        gH = _lib.scalar(f"{name}_g_and_H", *args, h, k)
+        Eo, go, Ho = O.constraint_gH(kind, X[i], X0[i], h, k, project=True)
+        assert np.abs(gH[:nd] - go).max() <= 1e-10 * np.abs(go).max()
+        assert np.abs(gH[nd:].reshape(nd, nd) - Ho).max() <= 1e-9 * np.abs(Ho).max()
+        assert np.linalg.eigvalsh(gH[nd:].reshape(nd, nd)).min() >= -1e-9 * np.abs(Ho).max()
+
+
+def test_barrier_and_classification_dropins():
+    """barrier_E/g/H against the closed form of distance.py:21-44 evaluated
+    in NumPy; PT_type / EE_type / PE_type / *_dist2 / M_threshold against the
+    oracle (pinned by brute-force closest distances)."""
+    h, k = 1e-4, 3.0
+    for d in (1e-9, 3e-7, 5e-5, 9.9e-5, 1e-4, 2e-4):
+        lg = np.log(d / h)
+        want = (-k * (d - h) ** 2 * lg, k * (-2 * (d - h) * lg - (d - h) ** 2 / d),
+                k * (-2 * lg - 4 * (d - h) / d + (d - h) ** 2 / d ** 2)) if d < h else (0.0, 0.0, 0.0)
+        for f, w in zip(("barrier_E", "barrier_g", "barrier_H"), want):
+            assert _lib.scalar(f, d, h, k)[0] == pytest.approx(w, rel=1e-12, abs=0.0)
+    rng = np.random.default_rng(4)
+    L = O.lib()
+    seen_pt, seen_ee = set(), set()
+    for _ in range(400):
+        P = rng.normal(size=(4, 3))
+        if rng.uniform() < 0.2:
+            P[3] = P[2] + (P[1] - P[0]) * rng.uniform(0.5, 2)  # parallel edges
+        pts = [O.dptr(np.ascontiguousarray(p)) for p in P]
+        t = int(_lib.scalar("PT_type", *P.ravel())[0])
+        assert t == L.orc_PT_type(*pts)
+        assert _lib.scalar("PT_dist2", *P.ravel(), t)[0] == L.orc_PT_dist2(*pts, t)
+        e = int(_lib.scalar("EE_type", *P.ravel())[0])
+        assert e == L.orc_EE_type(*pts)
+        assert _lib.scalar("EE_dist2", *P.ravel(), e)[0] == L.orc_EE_dist2(*pts, e)
+        assert int(_lib.scalar("PE_type", *P[:3].ravel())[0]) == L.orc_PE_type(*pts[:3])
+        assert _lib.scalar("M_threshold", *P.ravel())[0] == L.orc_M_threshold(*pts)
+        seen_pt.add(t)
+        seen_ee.add(e)
+    assert len(seen_pt) == 7 and len(seen_ee) >= 8
+
+
+def test_neohookean_tifunc_dropins():
+    """elasticity_* (neo_hookean.py:5-131): P and, in the pure-stretch cases
+    where no projection acts, dP/dF against the autodiff fixtures; dP/dF
+    elsewhere against the oracle; the sigma-space gradient and Hessian
+    against central differences of the sigma-space energy."""
+    AD = np.load(os.path.join(ROOT, "tests", "golden", "autodiff.npz"))
+    F, P, H = AD["nh_F"], AD["nh_P"], AD["nh_H_rowvec"]
+    la, mu = float(AD["nh_la"]), float(AD["nh_mu"])
+    perm = [3 * (a % 3) + a // 3 for a in range(9)]
+    for i in range(len(F)):
+        assert rel(_lib.scalar("elasticity_first_piola_kirchoff_stress", *F[i].ravel(), la, mu), P[i].ravel()) < 1e-10
+        D = _lib.scalar("elasticity_first_piola_kirchoff_stress_derivative", *F[i].ravel(), la, mu).reshape(9, 9)
+        Hc = np.empty((9, 9))
+        for a in range(9):
+            for b in range(9):
+                Hc[perm[a], perm[b]] = H[i][a, b]
+        assert rel(D, Hc) < 1e-8
+        s = np.linalg.svd(F[i], compute_uv=False)
+        assert _lib.scalar("elasticity_energy", *s, la, mu)[0] == pytest.approx(AD["nh_psi"][i], rel=1e-8, abs=1e-6)
+    rng = np.random.default_rng(6)
+    for _ in range(40):
+        Fm = np.eye(3) + rng.normal(size=(3, 3)) * 0.5
+        if np.linalg.det(Fm) < 0.1:
+            continue
+        D = _lib.scalar("elasticity_first_piola_kirchoff_stress_derivative", *Fm.ravel(), 4e5, 1e5).reshape(9, 9)
+        assert rel(D, O.nh_dPdF(Fm, 4e5, 1e5)) < 1e-9
+        s = np.sort(rng.uniform(0.5, 1.5, size=3))[::-1]
+        g = _lib.scalar("elasticity_gradient", *s, 4e5, 1e5)
+        Hs = _lib.scalar("elasticity_hessian", *s, 4e5, 1e5).reshape(3, 3)
+        eps = 1e-6
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = eps
+            fd = (_lib.scalar("elasticity_energy", *(s + e), 4e5, 1e5)[0]
+                  - _lib.scalar("elasticity_energy", *(s - e), 4e5, 1e5)[0]) / (2 * eps)
+            assert g[a] == pytest.approx(fd, rel=1e-6, abs=1e-3)
+            fdg = (_lib.scalar("elasticity_gradient", *(s + e), 4e5, 1e5)
+                   - _lib.scalar("elasticity_gradient", *(s - e), 4e5, 1e5)) / (2 * eps)
+            assert np.abs(Hs[:, a] - fdg).max() <= 1e-6 * np.abs(Hs).max()
