@@ -24,10 +24,15 @@ HBM for the whole run.
 
 Printed: one JSON line with steps/s, the roofline figures of the dominant
 kernel (HIP-event durations of its launches over --roofline-steps further
-steps of the same run, against its algorithmic bytes; the HBM traffic per launch measured with rocprofv3
-FETCH_SIZE / WRITE_SIZE, read from profiles/k_spmv_traffic_compressed_slab.md)
-and a CPU baseline (the C oracle, OpenMP, with a direct sparse solve) on a
-bounded sample of the same scene type.
+steps of the same run, against its algorithmic bytes; the HBM traffic per
+launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, read from
+profiles/k_spmv_traffic_compressed_slab.md), the kernel classes (SpMV,
+assembly with the FP64 VALU fraction of its per-tet pass, CCD with a probe of
+its narrow phase, broad phase), the whole 200-step trajectory of the value's
+form and of plain FP64 (full_trajectory), and a CPU baseline: the C oracle
+(OpenMP on every host core, a direct sparse solve per Newton iteration) on
+configs[0], two stacked cubes for 50 steps, beside the plain GPU path on the
+same scene.
 """
 from __future__ import annotations
 
@@ -45,7 +50,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TIMER_NAMES = ("spmv", "elastic_tet_pass", "ccd_query", "contact_query", "energy", "contact_assembly",
-               "pcg_vector", "grid_build", "assembly_gather", "spmv_residual")
+               "pcg_vector", "grid_build", "assembly_gather", "spmv_residual", "ccd_narrow")
+FP64_VALU_PEAK_TFLOPS = 78.6
 
 
 def log(msg):
@@ -107,17 +113,14 @@ VARIANTS = {
 WEIGHTS = os.path.join(ROOT, "fastipc_amd", "weights", "ws_slab.npz")
 
 
-def run_gpu(args, variant):
-    from fastipc_amd import _lib
-    from fastipc_amd import scene as S
+def make_sim(args, variant, sc):
+    """The solver of a variant on scene sc: (sim, warm-start hook, network,
+    weights metadata)."""
     from fastipc_amd.newton import NewtonIPC
     from fastipc_amd import warmstart as W
 
-    sc = S.compressed_slab(50, 80, 50, steps=args.warmup + args.steps + args.roofline_steps)
     fp32, ws, inexact = VARIANTS[variant]
-    precision = 1 if fp32 else 0
-    t0 = time.perf_counter()
-    sim = NewtonIPC(sc, precision=precision, pcg_tol=args.pcg_tol,
+    sim = NewtonIPC(sc, precision=1 if fp32 else 0, pcg_tol=args.pcg_tol,
                     pcg_tol_search=args.pcg_tol_search if inexact else None)
     hook, net, warm_meta = None, None, None
     if ws == "learned":
@@ -132,6 +135,18 @@ def run_gpu(args, variant):
         net = W.WarmStartMLP(seed=0)
         net.set_params(np.zeros(net.nparams, np.float32))
         hook = W.LearnedWarmStart(net, W.scale_for(sc))
+    return sim, hook, net, warm_meta
+
+
+def run_gpu(args, variant):
+    from fastipc_amd import _lib
+    from fastipc_amd import scene as S
+
+    sc = S.compressed_slab(50, 80, 50, steps=args.warmup + args.steps + args.roofline_steps)
+    fp32, ws, inexact = VARIANTS[variant]
+    precision = 1 if fp32 else 0
+    t0 = time.perf_counter()
+    sim, hook, net, warm_meta = make_sim(args, variant, sc)
     setup_s = time.perf_counter() - t0
     newton = []
     accepted = 0
@@ -201,21 +216,34 @@ def run_gpu(args, variant):
             "avg_launch_us": round(kt[dom]["us_avg"], 2), "launches": kt[dom]["launches"]}
     others = {k: {"achieved_GBs": round(nbytes[k] / (kt[k]["us_avg"] * 1e-6) / 1e9, 1) if kt[k]["us_avg"] else None,
                   "avg_launch_us": round(kt[k]["us_avg"], 2)} for k in nbytes}
+    # FP64 VALU fraction of the per-tet pass (SVD, projected dP/dF, 12x12
+    # blocks): its FP64 operations per launch from the committed rocprofv3
+    # counters over its live event-timed duration
+    flops = valu_flops()
+    if flops.get("k_elastic") and kt["elastic_tet_pass"]["us_avg"]:
+        f = flops["k_elastic"]
+        tf = f / (kt["elastic_tet_pass"]["us_avg"] * 1e-6) / 1e12
+        others["elastic_assembly"].update({"tet_pass_fp64_flops_per_launch": f, "tet_pass_fp64_tflops": round(tf, 2),
+                                           "tet_pass_valu_frac": round(tf / FP64_VALU_PEAK_TFLOPS, 4)})
     # CCD pass (point-triangle and edge-edge candidate queries on the swept
-    # boxes, then the narrow phases: four timed launches per pass); its
-    # compulsory unique bytes are x and the direction at the surface vertices,
-    # the primitive indices and 16 bytes per candidate pair of the last pass
+    # boxes, then the narrow phases: two timed query and two narrow-phase
+    # launches per pass); its compulsory unique bytes are x and the direction
+    # at the surface vertices, the primitive indices and 16 bytes per
+    # candidate pair of the last pass
     pairs = np.zeros(2, np.int32)
     _lib.call("fipc_ccd_counts", sim.ctx, _lib.iptr(pairs))
     s_ = sim.setup
     nbp, nbe, nbt = len(s_.bp), len(s_.be), len(s_.bt)
     ccd_bytes = nbp * 48 + nbp * 4 + nbe * 8 + nbt * 12 + int(pairs.sum()) * 16
-    passes = kt["ccd_query"]["launches"] // 4
+    passes = kt["ccd_query"]["launches"] // 2
     if passes:
-        pass_us = kt["ccd_query"]["ms_total"] / passes * 1e3
+        pass_us = (kt["ccd_query"]["ms_total"] + kt["ccd_narrow"]["ms_total"]) / passes * 1e3
         others["ccd"] = {"achieved_GBs": round(ccd_bytes / (pass_us * 1e-6) / 1e9, 1), "avg_pass_us": round(pass_us, 2),
+                         "hbm_frac": round(ccd_bytes / (pass_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 5),
                          "unique_bytes_per_pass": ccd_bytes, "candidate_pairs_last_pass": [int(p) for p in pairs],
-                         "bound": "FP64 VALU / latency"}
+                         "bound": "latency of the grid queries (no pair reaches the narrow phase in these steps)"}
+    others["ccd_narrow_phase"] = ccd_narrow_probe(sim, flops)
+    others["broad_phase"] = broad_phase_probe(sim)
     res = {
         "variant": variant,
         "warm_start_kept": accepted,
@@ -238,7 +266,182 @@ def run_gpu(args, variant):
     return res
 
 
+def valu_flops():
+    """FP64 operations per launch of the FP64-VALU-bound kernels, from the
+    committed rocprofv3 counter summary (lines 'fp64_flops_per_launch:
+    <kernel>=<N>'): 64 lanes x (ADD + MUL + TRANS + 2 FMA) FP64 VALU
+    instructions."""
+    import re
+    path = os.path.join(ROOT, "profiles", "valu_fp64_compressed_slab.md")
+    try:
+        txt = open(path).read()
+    except OSError:
+        return {}
+    return {k: int(v) for k, v in re.findall(r"fp64_flops_per_launch:\s*(\w+)=(\d+)", txt)}
+
+
+def ccd_narrow_probe(sim, flops):
+    """The CCD narrow phase with work to do: the benchmark state with a
+    seeded smooth direction of amplitude 2 dHat on the free vertices (the
+    benchmark's own directions leave no pair below the lower bound), so
+    that the pairs near the contact surfaces reach the root finding.  Pairs
+    per second, the narrow phase's event-timed duration and its FP64 VALU
+    fraction (operations per launch from the committed counters of this same
+    probe)."""
+    from fastipc_amd import _lib
+    rng = np.random.default_rng(2024)
+    x = sim.x
+    free = ~sim.setup.dirichlet.astype(bool)
+    k = rng.normal(size=(3, 3)) * 2.0
+    ph = rng.uniform(0, 2 * np.pi, size=3)
+    p = np.zeros_like(x)
+    p[free] = 2.0 * sim.setup.dHat * np.sin(x[free] @ k + ph)
+    sim.set_vector(1, p)
+    _lib.call("fipc_enable_timing", sim.ctx, 1)
+    reps = 5
+    for _ in range(reps):
+        alpha = sim.compute_intersection_free_step_size()
+    nt_ = len(TIMER_NAMES)
+    ms = np.zeros(nt_)
+    la = (C.c_long * nt_)()
+    _lib.call("fipc_kernel_times", sim.ctx, nt_, _lib.dptr(ms), la)
+    _lib.call("fipc_enable_timing", sim.ctx, 0)
+    pairs = np.zeros(2, np.int32)
+    _lib.call("fipc_ccd_counts", sim.ctx, _lib.iptr(pairs))
+    i = TIMER_NAMES.index("ccd_narrow")
+    us = ms[i] / la[i] * 1e3 if la[i] else 0.0  # per launch (two launches per pass: PT, EE)
+    out = {"pairs_per_pass": [int(v) for v in pairs], "alpha": alpha, "avg_launch_us": round(us, 2),
+           "pairs_per_s": round(float(pairs.sum()) / (2 * us * 1e-6), 1) if us else None,
+           "direction": "seeded smooth field, amplitude 2 dHat, on the benchmark state"}
+    f = flops.get("k_ccd_pt", 0) + flops.get("k_ccd_ee", 0)
+    if f and us:
+        tf = f / (2 * us * 1e-6) / 1e12
+        out.update({"fp64_flops_per_pass": f, "fp64_tflops": round(tf, 3), "valu_frac": round(tf / FP64_VALU_PEAK_TFLOPS, 5)})
+    return out
+
+
+def broad_phase_probe(sim, reps=10):
+    """One constraint search (NEWTON_IPC.py:337-357) at the benchmark state
+    with fresh grids (the state is rewritten, which invalidates them): host
+    wall time per pass including its two synchronisations, against the
+    compulsory bytes: the surface positions and primitive indices read, 8
+    bytes per broad-phase candidate pair and the rows written."""
+    x = sim.x
+    ts = []
+    for _ in range(reps):
+        sim.set_state(x)
+        t0 = time.perf_counter()
+        counts = sim.find_constraints()
+        ts.append(time.perf_counter() - t0)
+    # one more pass, untimed, to count the candidates
+    sim.search_stats(True)
+    sim.set_state(x)
+    sim.find_constraints()
+    cand = sim.search_counts()
+    sim.search_stats(False)
+    s_ = sim.setup
+    nbp, nbe, nbt = len(s_.bp), len(s_.be), len(s_.bt)
+    width = (2, 3, 4, 4, 4, 4, 4)
+    nbytes = nbp * (24 + 4) + nbe * 8 + nbt * 12 + 8 * sum(cand) + sum(4 * w * n for w, n in zip(width, counts))
+    t = float(np.median(ts))
+    gbs = nbytes / t / 1e9
+    return {"pass_us": round(t * 1e6, 1), "candidates": cand, "rows": counts, "unique_bytes_per_pass": nbytes,
+            "achieved_GBs": round(gbs, 2), "hbm_frac": round(gbs / HBM_PEAK_GBS, 6),
+            "bound": "launch and synchronisation latency (about ten launches, two host waits)"}
+
+
+def run_full_trajectory(args, variant, nsteps=200):
+    """The benchmark scene's whole 200-step trajectory (both plates in
+    contact from about step 80) with a fresh solver: steps 2..200 timed (the
+    first is the warm-up, as the reference's Timer_Print excludes it), every
+    step checked for intersections."""
+    from fastipc_amd import _lib
+    from fastipc_amd import scene as S
+
+    sc = S.compressed_slab(50, 80, 50, steps=nsteps)
+    sim, hook, net, _ = make_sim(args, variant, sc)
+    sim.step(warm_start=hook)
+    if sim.check_collision():
+        raise RuntimeError("intersection after a step")
+    newton, pcg = [], []
+    _lib.call("fipc_sync", sim.ctx)
+    t0 = time.perf_counter()
+    for _ in range(1, nsteps):
+        st = sim.step(warm_start=hook)
+        newton.append(st.newton_iters)
+        pcg.extend(st.pcg_iters)
+        if sim.check_collision():
+            raise RuntimeError("intersection after a step")
+    _lib.call("fipc_sync", sim.ctx)
+    wall = time.perf_counter() - t0
+    counts = sim.find_constraints()
+    rows = sim.constraints(sort=False)
+    nslab = sim.nv - 16
+    self_rows = int(sum(np.all(r < nslab, axis=1).sum() for r in rows.values() if len(r)))
+    newton = np.asarray(newton)
+    out = {"steps_per_s": round((nsteps - 1) / wall, 4), "timed_steps": nsteps - 1,
+           "newton_iters_per_step": round(float(newton.mean()), 3),
+           "newton_iters_per_step_by_quarter": [round(float(q.mean()), 2) for q in np.array_split(newton, 4)],
+           "pcg_iters_per_newton": round(float(np.mean(pcg)), 1),
+           "contacts_last_step": dict(zip(("PP", "PE", "PT", "EE", "EEM", "PPM", "PEM"), counts)),
+           "slab_slab_rows_last_step": self_rows, "intersections": 0}
+    log(f"[{variant}] full trajectory: {json.dumps(out)}")
+    sim.close()
+    if net is not None:
+        net.close()
+    return out
+
+
 def run_cpu_baseline(args):
+    """The oracle (strict-IEEE C, OpenMP on every host core, CSR + a direct
+    sparse solve per Newton iteration: the reference's structure) on
+    configs[0] of the benchmark definition: two stacked 10^3-vertex cubes
+    dropped on a ground box, 50 steps; steps/s over steps 2..50 (the first
+    is excluded, as the reference's Timer_Print does)."""
+    from fastipc_amd import scene as S
+    from oracle import lib as OL
+    from oracle.newton import OracleIPC, SOLVER
+
+    sc = S.config(0)
+    o = OracleIPC(sc)
+    o.step()
+    t0 = time.perf_counter()
+    newton = 0
+    for _ in range(sc.steps - 1):
+        newton += o.step()["newton_iters"]
+    wall = time.perf_counter() - t0
+    return {"value": round((sc.steps - 1) / wall, 4), "unit": "steps/s", "cores": OL.lib().orc_num_threads(),
+            "kind": "port",
+            "sample": f"two stacked cubes (configs[0]: {sc.n_verts} verts, {sc.n_tets} tets), {sc.steps} steps, "
+                      f"steps 2-{sc.steps} timed ({wall:.1f} s, {newton / (sc.steps - 1):.2f} Newton iterations "
+                      f"per step), direct solver {SOLVER}; a different, far smaller scene than the GPU value's"}
+
+
+def run_gpu_config0(args):
+    """The plain FP64 GPU path on the CPU baseline's scene,
+    counted the same way, for a same-scene comparison."""
+    from fastipc_amd import _lib
+    from fastipc_amd import scene as S
+    from fastipc_amd.newton import NewtonIPC
+
+    sc = S.config(0)
+    sim = NewtonIPC(sc, pcg_tol=args.pcg_tol)
+    sim.step()
+    _lib.call("fipc_sync", sim.ctx)
+    t0 = time.perf_counter()
+    newton = 0
+    for _ in range(sc.steps - 1):
+        newton += sim.step().newton_iters
+        if sim.check_collision():
+            raise RuntimeError("intersection after a step")
+    _lib.call("fipc_sync", sim.ctx)
+    wall = time.perf_counter() - t0
+    sim.close()
+    return {"steps_per_s": round((sc.steps - 1) / wall, 3), "newton_iters_per_step": round(newton / (sc.steps - 1), 2),
+            "variant": "plain"}
+
+
+def run_cpu_slab_extrapolation(args):
     """The oracle (strict-IEEE C, OpenMP, CSR + direct solve) on a bounded
     sample: the same slab scene at 10 x 16 x 10 vertices, steps after the
     plates touch, scaled to the full slab linearly in the tet count
@@ -285,6 +488,10 @@ def main():
     ap.add_argument("--pcg-tol-search", type=float, default=2e-2,
                     help="PCG tolerance of the search directions of the +inexact variants")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-full-trajectory", action="store_true",
+                    help="skip the 200-step runs of the value's form and of plain FP64")
+    ap.add_argument("--no-cpu-slab", action="store_true",
+                    help="skip the extrapolated CPU rate of the slab (10x16x10 sample scaled by tets)")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-warmup", type=int, default=6)
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
@@ -311,7 +518,17 @@ def main():
                      "newton_iters_per_step": round(r["newton_iters_per_step"], 2),
                      "pcg_iters_per_newton": round(r["pcg_iters_per_newton"], 1),
                      "warm_start_kept": r["warm_start_kept"]}
+    full = {}
+    if not args.no_full_trajectory:
+        for v in dict.fromkeys((args.mode, "plain")):
+            full[v] = run_full_trajectory(args, v)
+        if "plain" in full and args.mode != "plain":
+            full["speedup_vs_plain_fp64"] = round(full[args.mode]["steps_per_s"] / full["plain"]["steps_per_s"], 4)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(args)
+    if cpu is not None:
+        cpu["gpu_plain_fp64_same_scene"] = run_gpu_config0(args)
+        if not args.no_cpu_slab:
+            cpu["slab_extrapolation"] = run_cpu_slab_extrapolation(args)
     fp32, ws, inexact = VARIANTS[args.mode]
     line = {
         "metric": "sim steps/s at 1M tets, warm-start+FP32 vs plain FP64 GPU at equal residual",
@@ -342,6 +559,9 @@ def main():
         # directions too: what the warm start and the FP32 Hessian add
         "speedup_vs_plain_fp64_inexact": (round(g["steps_per_s"] / others["plain+inexact"]["steps_per_s"], 4)
                                           if "plain+inexact" in others else None),
+        # the whole 200-step trajectory of the benchmark scene (steps 2-200),
+        # value's form and plain FP64, fresh solvers in the same run
+        "full_trajectory": full or None,
         "roofline": g["roofline"],
         "cpu_baseline": cpu,
         "kernel_classes": g["kernel_classes"],

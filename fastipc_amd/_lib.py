@@ -55,6 +55,8 @@ _SIGS = {
     "fipc_energy": [_VP, _P, _P],
     "fipc_ccd_alpha": [_VP, _D, C.c_int, _P, _P],
     "fipc_ccd_counts": [_VP, _IP],
+    "fipc_search_stats": [_VP, C.c_int],
+    "fipc_search_counts": [_VP, _IP],
     "fipc_save_prev": [_VP],
     "fipc_apply_sol": [_VP, _D],
     "fipc_check_collision": [_VP, _IP],

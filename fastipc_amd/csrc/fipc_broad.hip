@@ -12,7 +12,10 @@
 // buffers from capacities and never wait for the host; each operation
 // synchronises once at its end and is redone when a capacity was exceeded.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_merge_sort.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
 #include <algorithm>
 #include <vector>
 #include "fipc_ctx.h"
@@ -213,10 +216,10 @@ static int scan_inplace(fipc_ctx* c, int* d, int n, hipStream_t st = nullptr)
     const hipStream_t s = st ? st : c->s;
     DVec<unsigned char>& tmp = s == c->s ? c->cub_tmp : c->cub_tmp2;
     size_t bytes = 0;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, d, d, n, s));
+    HIPCHK(rocprim::exclusive_scan(nullptr, bytes, d, d, 0, (size_t)n, rocprim::plus<int>(), s));
     if (bytes > tmp.cap) STCHK(ensure(tmp, bytes));
     bytes = tmp.cap;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, d, d, n, s));
+    HIPCHK(rocprim::exclusive_scan(tmp.p, bytes, d, d, 0, (size_t)n, rocprim::plus<int>(), s));
     return FIPC_OK;
 }
 
@@ -306,8 +309,15 @@ static int bp_check(fipc_ctx* c, const int* hflag, bool* retry)
 }
 
 // ---- constraint search ----------------------------------------------------
+// one atomic per wavefront: adds the number of active lanes to *cnt
+__device__ __forceinline__ void wave_count(int* cnt)
+{
+    unsigned long long m = __ballot(1);
+    if (__lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(cnt, (int)__popcll(m));
+}
 struct RowSink {
     int* cnt;                       // 7 raw counters
+    int* cand;                      // broad-phase candidates (PT, EE), or null when not counted
     long long cap[NTYPES];
     unsigned long long* kpp;        // PP keys (a << 21 | b)
     unsigned long long* kpe;        // PE keys (p << 42 | e0 << 21 | e1)
@@ -344,6 +354,7 @@ struct VisitPT {
         v3 mx = vmax(vmax(T0, T1), T2), mn = vmin(vmin(T0, T1), T2);
         if (!(P.x < mx.x + dHat && P.y < mx.y + dHat && P.z < mx.z + dHat)) return;
         if (!(mn.x - dHat < P.x && mn.y - dHat < P.y && mn.z - dHat < P.z)) return;
+        if (out.cand) wave_count(out.cand);
         int k = PT_type(P, T0, T1, T2);
         if (!(PT_dist2(P, T0, T1, T2, k) < dHat2)) return;
         switch (k) {
@@ -373,6 +384,7 @@ struct VisitEE {
         v3 mxa = vmax(A0, A1), mna = vmin(A0, A1), mxb = vmax(B0, B1), mnb = vmin(B0, B1);
         if (!(mna.x < mxb.x + dHat && mna.y < mxb.y + dHat && mna.z < mxb.z + dHat)) return;
         if (!(mnb.x - dHat < mxa.x && mnb.y - dHat < mxa.y && mnb.z - dHat < mxa.z)) return;
+        if (out.cand) wave_count(out.cand + 1);
         int k = EE_type(A0, A1, B0, B1);
         if (!(EE_dist2(A0, A1, B0, B1, k) < dHat2)) return;
         double cn2 = EECN2(A0, A1, B0, B1);
@@ -420,14 +432,43 @@ static int dedup(fipc_ctx* c, const unsigned long long* keys, int n, int end_bit
                  unsigned long long* uniq, int* d_n)
 {
     size_t b1 = 0, b2 = 0;
-    HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, b1, keys, sorted, n, 0, end_bit, c->s));
-    HIPCHK(hipcub::DeviceSelect::Unique(nullptr, b2, sorted, uniq, d_n, n, c->s));
+    HIPCHK(rocprim::radix_sort_keys(nullptr, b1, keys, sorted, n, 0, end_bit, c->s));
+    HIPCHK(rocprim::unique(nullptr, b2, sorted, uniq, d_n, (size_t)n, rocprim::equal_to<unsigned long long>(), c->s));
     size_t need = std::max(b1, b2);
     if (need > c->cub_tmp.cap) STCHK(ensure(c->cub_tmp, need));
     b1 = c->cub_tmp.cap;
-    HIPCHK(hipcub::DeviceRadixSort::SortKeys(c->cub_tmp.p, b1, keys, sorted, n, 0, end_bit, c->s));
+    HIPCHK(rocprim::radix_sort_keys(c->cub_tmp.p, b1, keys, sorted, n, 0, end_bit, c->s));
     b2 = c->cub_tmp.cap;
-    HIPCHK(hipcub::DeviceSelect::Unique(c->cub_tmp.p, b2, sorted, uniq, d_n, n, c->s));
+    HIPCHK(rocprim::unique(c->cub_tmp.p, b2, sorted, uniq, d_n, (size_t)n, rocprim::equal_to<unsigned long long>(),
+                           c->s));
+    return FIPC_OK;
+}
+
+// lexicographic order of four-index rows (np.unique order)
+struct Row4Less {
+    __device__ __forceinline__ bool operator()(const int4& a, const int4& b) const
+    {
+        if (a.x != b.x) return a.x < b.x;
+        if (a.y != b.y) return a.y < b.y;
+        if (a.z != b.z) return a.z < b.z;
+        return a.w < b.w;
+    }
+};
+// sorts the n rows of a four-index type in place (through the scratch
+// rows_tmp): the row order, and so every later sum over rows, no longer
+// depends on the order the search appended them in
+static int sort_rows4(fipc_ctx* c, DVec<int>& rows, int n)
+{
+    if (n < 2) return FIPC_OK;
+    STCHK(ensure(c->rows_tmp, (size_t)n * 4));
+    const int4* in = reinterpret_cast<const int4*>(rows.p);
+    int4* out = reinterpret_cast<int4*>(c->rows_tmp.p);
+    size_t b = 0;
+    HIPCHK(rocprim::merge_sort(nullptr, b, in, out, (size_t)n, Row4Less(), c->s));
+    if (b > c->cub_tmp.cap) STCHK(ensure(c->cub_tmp, b));
+    b = c->cub_tmp.cap;
+    HIPCHK(rocprim::merge_sort(c->cub_tmp.p, b, in, out, (size_t)n, Row4Less(), c->s));
+    std::swap(rows, c->rows_tmp);
     return FIPC_OK;
 }
 
@@ -443,9 +484,12 @@ static int bits_for(int n)
 static int find_once(fipc_ctx* c, int* counts, int* hflag)
 {
     STCHK(bp_begin(c));
-    HIPCHK(hipMemsetAsync(c->counters.p, 0, NTYPES * 4, c->s));
+    HIPCHK(hipMemsetAsync(c->counters.p, 0, 12 * 4, c->s));
     RowSink sink;
     sink.cnt = c->counters.p;
+    // one same-address atomic per wavefront and candidate test would
+    // serialise the queries: the candidates are counted only on request
+    sink.cand = c->search_stats ? c->counters.p + 10 : nullptr;
     for (int t = 0; t < NTYPES; t++) { sink.cap[t] = c->cap[t]; sink.rows[t] = c->rows[t].p; }
     sink.kpp = c->keys_pp.p;
     sink.kpe = c->keys_pe.p;
@@ -461,7 +505,7 @@ static int find_once(fipc_ctx* c, int* counts, int* hflag)
     BoxFn qedges{c->x.p, nullptr, c->be.p, 1, c->dHat};
     VisitEE vee{c->x.p, c->x0.p, c->be.p, c->dHat, c->dHat2, sink};
     STCHK(run_query(c, 1, c->nbe, qedges, vee, TM_FIND));
-    STCHK(d2h_async(c, counts, c->counters.p, NTYPES * 4));
+    STCHK(d2h_async(c, counts, c->counters.p, 12 * 4));
     STCHK(d2h_async(c, hflag, c->flag.p, 12));
     STCHK(d2h_wait(c));
     return FIPC_OK;
@@ -470,7 +514,7 @@ static int find_once(fipc_ctx* c, int* counts, int* hflag)
 extern "C" int fipc_find_constraints(fipc_ctx* c, int* counts_out)
 {
     if (!c || !c->nv) return FIPC_ERR_STATE;
-    int raw[NTYPES];
+    int raw[12];
     for (int attempt = 0;; attempt++) {
         if (attempt == 6) return FIPC_ERR_CAPACITY;
         STCHK(ensure(c->keys_pp, c->cap[0]));
@@ -486,6 +530,8 @@ extern "C" int fipc_find_constraints(fipc_ctx* c, int* counts_out)
         if (!grow) break;
     }
     for (int t = 0; t < NTYPES; t++) c->raw_count[t] = raw[t];
+    c->cand_last[0] = c->search_stats ? raw[10] : -1;
+    c->cand_last[1] = c->search_stats ? raw[11] : -1;
     int bits = bits_for(c->nv);
     // sorted and unique keys of PP then PE share keys_tmp
     size_t npp_raw = raw[0], npe_raw = raw[1];
@@ -504,6 +550,7 @@ extern "C" int fipc_find_constraints(fipc_ctx* c, int* counts_out)
         k_decode_pe<<<stride_grid(npe_raw), TPB, 0, c->s>>>(c->nuniq.p + 1, k2 + npe_raw, c->rows[1].p);
     }
     HIPCHK(hipGetLastError());
+    for (int t = 2; t < NTYPES; t++) STCHK(sort_rows4(c, c->rows[t], raw[t]));
     int nu[2];
     STCHK(d2h_async(c, nu, c->nuniq.p, 8));
     STCHK(d2h_wait(c));
@@ -678,22 +725,22 @@ extern "C" int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* al
         BoxFn qedges{c->x.p, c->sol.p, c->be.p, 1, c->dHat};
         CandCCDEE vee{c->x.p, c->sol.p, c->be.p, c->dHat, eta, PairSink{pc + 1, cap, c->ccd_pairs[1].p}};
         STCHK(run_query(c, 1, c->nbe, qedges, vee, TM_CCD, c->s2));
-        STCHK(fipc_tm_begin(c, TM_CCD, c->s2));
+        STCHK(fipc_tm_begin(c, TM_NARROW, c->s2));
         k_ccd_ee<<<stride_grid(cap), TPB, 0, c->s2>>>(pc + 1, cap, c->ccd_pairs[1].p, c->x.p, c->sol.p, c->be.p, eta,
                                                       c->amin.p + 1);
         HIPCHK(hipGetLastError());
-        STCHK(fipc_tm_end(c, TM_CCD, c->s2));
+        STCHK(fipc_tm_end(c, TM_NARROW, c->s2));
         HIPCHK(hipEventRecord(c->ev_contact, c->s2));
         BoxFn pts{c->x.p, c->sol.p, c->bp.p, 0, c->dHat};
         STCHK(grid_build(c, 0, c->nbp, pts));
         BoxFn tris{c->x.p, c->sol.p, c->bt.p, 2, 0.0};
         CandCCDPT vpt{c->x.p, c->sol.p, c->bp.p, c->bt.p, c->dHat, eta, PairSink{pc, cap, c->ccd_pairs[0].p}};
         STCHK(run_query(c, 0, c->nbt, tris, vpt, TM_CCD));
-        STCHK(fipc_tm_begin(c, TM_CCD));
+        STCHK(fipc_tm_begin(c, TM_NARROW));
         k_ccd_pt<<<stride_grid(cap), TPB, 0, c->s>>>(pc, cap, c->ccd_pairs[0].p, c->x.p, c->sol.p, c->bp.p, c->bt.p, eta,
                                                      c->amin.p);
         HIPCHK(hipGetLastError());
-        STCHK(fipc_tm_end(c, TM_CCD));
+        STCHK(fipc_tm_end(c, TM_NARROW));
         if (inversion) STCHK(fipc_inversion_alpha(c, c->amin.p + 2));
         HIPCHK(hipStreamWaitEvent(c->s, c->ev_contact, 0));
         unsigned long long out[3];
@@ -717,6 +764,21 @@ extern "C" int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* al
     }
     if (parts) { parts[0] = a[0]; parts[1] = a[1]; parts[2] = a[2]; }
     *alpha = std::min(std::min(a[0], a[1]), a[2]);
+    return FIPC_OK;
+}
+
+extern "C" int fipc_search_stats(fipc_ctx* c, int on)
+{
+    if (!c) return FIPC_ERR_ARG;
+    c->search_stats = on != 0;
+    return FIPC_OK;
+}
+
+extern "C" int fipc_search_counts(fipc_ctx* c, int* cand)
+{
+    if (!c || !cand) return FIPC_ERR_ARG;
+    cand[0] = c->cand_last[0];
+    cand[1] = c->cand_last[1];
     return FIPC_OK;
 }
 
@@ -795,3 +857,4 @@ extern "C" int fipc_batch_ccd(int type, int n, const double* pts, const double* 
     (void)hipFree(dp); (void)hipFree(dd); (void)hipFree(d2); (void)hipFree(dt);
     return FIPC_OK;
 }
+

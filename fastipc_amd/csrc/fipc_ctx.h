@@ -82,8 +82,9 @@ struct Grid {
 
 // HIP-event timing of single kernels: pairs are recorded on the context
 // stream without synchronising and folded into totals when read
-#define FIPC_NTIMERS 10
-enum { TM_SPMV = 0, TM_ELASTIC, TM_CCD, TM_FIND, TM_ENERGY, TM_CONTACT, TM_PCGVEC, TM_GRID, TM_GATHER, TM_SPMV_RES };
+#define FIPC_NTIMERS 11
+enum { TM_SPMV = 0, TM_ELASTIC, TM_CCD, TM_FIND, TM_ENERGY, TM_CONTACT, TM_PCGVEC, TM_GRID, TM_GATHER, TM_SPMV_RES,
+       TM_NARROW };
 struct KernelTimer {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     double total_ms = 0.0;
@@ -122,10 +123,16 @@ struct fipc_ctx {
     int count[NTYPES] = {0};
     int raw_count[NTYPES] = {0};
     DVec<int> rows[NTYPES];
+    DVec<int> rows_tmp;    // the sort of the four-index types writes here, then swaps
     DVec<double> cH[NTYPES];
+    // FP32 Hessian: FP32 copies of the contact rows' projected Hessians for
+    // the FP32 iterations, and per row a flag (1) for the rows closer than
+    // 1e-2 dHat, whose FP64 Hessian those iterations keep using
+    DVec<float> cHf[NTYPES];
+    DVec<unsigned char> c64[NTYPES];
     DVec<unsigned long long> keys_pp, keys_pe, keys_tmp;
     DVec<int> counters;    // NTYPES raw append counters + misc
-    DVec<unsigned char> cub_tmp, cub_tmp2;  // scan / sort scratch of the first and second stream
+    DVec<unsigned char> cub_tmp, cub_tmp2;  // rocPRIM scan / sort scratch of the first and second stream
     size_t cub_bytes = 0;
     DVec<int> nuniq;
     // preconditioner + PCG
@@ -163,6 +170,8 @@ struct fipc_ctx {
     int ccd_cap = 0;
     DVec<int2> ccd_pairs[2];
     int ccd_last[2] = {0, 0};  // candidate pairs (PT, EE) of the last fipc_ccd_alpha
+    int cand_last[2] = {-1, -1}; // broad-phase candidates (PT, EE) of the last fipc_find_constraints
+    bool search_stats = false;   // count them (fipc_search_stats)
     // broad phase grids
     Grid grid[2];                         // [0] boundary points, [1] boundary edges
     unsigned long long x_version = 1;     // incremented by every write of x

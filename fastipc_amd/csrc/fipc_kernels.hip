@@ -328,7 +328,7 @@ __device__ __forceinline__ void contact_row(int n, const int* __restrict__ rows,
                                             double* __restrict__ Hout, double* rhs, double* cdiag, double* Eout,
                                             double* gout, int project, int accumulate,
                                             const unsigned char* __restrict__ dmask, int max_sweeps, int gid,
-                                            double* lds)
+                                            double* lds, float* __restrict__ Hf, unsigned char* __restrict__ c64)
 {
     using KI = KindInfo<KIND>;
     constexpr int N = KI::N, G = KI::G;
@@ -472,6 +472,14 @@ __device__ __forceinline__ void contact_row(int n, const int* __restrict__ rows,
     double* Hr = Hout + (size_t)r * N * N + (size_t)l * N;
 #pragma unroll
     for (int k = 0; k < N; k++) Hr[k] = out[k];
+    if (Hf) {
+        // FP32 copy for the FP32 iterations; rows closer than 1e-2 dHat
+        // (squared distance below 1e-4 dHat^2) are flagged to stay FP64
+        float* Hfr = Hf + (size_t)r * N * N + (size_t)l * N;
+#pragma unroll
+        for (int k = 0; k < N; k++) Hfr[k] = (float)out[k];
+        if (l == 0) c64[r] = constraint_dist(KIND, px) < 1e-4 * dHat2 ? 1 : 0;
+    }
     const int L = l / 3, i = l % 3;
     int vid = id[0];
     double gl = g[0];
@@ -502,6 +510,8 @@ __device__ __forceinline__ void contact_row(int n, const int* __restrict__ rows,
 struct ContactBatch {
     const int* rows[NTYPES];
     double* H[NTYPES];
+    float* Hf[NTYPES];            // null unless FP32 copies are wanted
+    unsigned char* c64[NTYPES];
     int n[NTYPES];
     int boff[NTYPES + 1];
 };
@@ -518,7 +528,7 @@ __global__ void __launch_bounds__(TPB) k_contact(ContactBatch cb, const double* 
 #define CONTACT_CASE(K)                                                                                            \
     case K:                                                                                                         \
         contact_row<K>(cb.n[K], cb.rows[K], x, x0, dHat2, kappa, cb.H[K], rhs, cdiag, Eout, gout, project,         \
-                       accumulate, dmask, max_sweeps, gid, lds);                                                    \
+                       accumulate, dmask, max_sweeps, gid, lds, cb.Hf[K], cb.c64[K]);                               \
         break;
     switch (t) {
         CONTACT_CASE(0)
@@ -615,11 +625,34 @@ static inline int spmv_grid(int nv)
 struct ContactSet {
     const int* rows[NTYPES];
     const double* H[NTYPES];
+    const float* Hf[NTYPES];            // FP32 copies (FP32 iterations), or null
+    const unsigned char* c64[NTYPES];   // rows that keep their FP64 Hessian
     int n[NTYPES];
     int boff[NTYPES + 1];
 };
+template <int KIND, class V, class H>
+__device__ __forceinline__ double contact_row_dot(const H* __restrict__ Hr, const double (&pv)[3 * KindInfo<KIND>::K],
+                                                  const V* __restrict__ p, double* qc, int v)
+{
+    constexpr int N = KindInfo<KIND>::N;
+    double pq = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < N; c++) s += (double)Hr[i * N + c] * pv[c];
+        atomicAdd(&qc[3 * v + i], s);
+        pq += (double)p[3 * v + i] * s;
+    }
+    return pq;
+}
+// contact rows of one type: lane `lane` of row r owns the three Hessian rows
+// of the row's vertex `lane`.  With Hf set (FP32 iterations) a row uses its
+// FP32 copy unless flagged in c64.
 template <int KIND, class V>
 __device__ __forceinline__ double contact_rows_spmv(int n, const int* __restrict__ rows, const double* __restrict__ Hc,
+                                                    const float* __restrict__ Hf,
+                                                    const unsigned char* __restrict__ c64,
                                                     const V* __restrict__ p, double* qc,
                                                     const unsigned char* __restrict__ dmask, int gid)
 {
@@ -634,19 +667,11 @@ __device__ __forceinline__ double contact_rows_spmv(int n, const int* __restrict
         int v = rows[W * r + a];
         pv[3 * a] = p[3 * v]; pv[3 * a + 1] = p[3 * v + 1]; pv[3 * a + 2] = p[3 * v + 2];
     }
-    const double* Hr = Hc + (size_t)r * N * N + lane * 3 * N;
     int v = rows[W * r + lane];
     if (dmask[v]) return 0.0;
-    double pq = 0.0;
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        double s = 0.0;
-#pragma unroll
-        for (int c = 0; c < N; c++) s += Hr[i * N + c] * pv[c];
-        atomicAdd(&qc[3 * v + i], s);
-        pq += (double)p[3 * v + i] * s;
-    }
-    return pq;
+    const size_t off = (size_t)r * N * N + lane * 3 * N;
+    if (Hf && !c64[r]) return contact_row_dot<KIND, V, float>(Hf + off, pv, p, qc, v);
+    return contact_row_dot<KIND, V, double>(Hc + off, pv, p, qc, v);
 }
 template <class T, class V>
 __global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ rowptr, const int* __restrict__ col,
@@ -734,13 +759,13 @@ __global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ ro
         while (t < NTYPES - 1 && cb >= cs.boff[t + 1]) t++;
         int gid = (cb - cs.boff[t]) * TPB + (int)threadIdx.x;
         switch (t) {
-        case 0: pq = contact_rows_spmv<0, V>(cs.n[0], cs.rows[0], cs.H[0], p, qc, dmask, gid); break;
-        case 1: pq = contact_rows_spmv<1, V>(cs.n[1], cs.rows[1], cs.H[1], p, qc, dmask, gid); break;
-        case 2: pq = contact_rows_spmv<2, V>(cs.n[2], cs.rows[2], cs.H[2], p, qc, dmask, gid); break;
-        case 3: pq = contact_rows_spmv<3, V>(cs.n[3], cs.rows[3], cs.H[3], p, qc, dmask, gid); break;
-        case 4: pq = contact_rows_spmv<4, V>(cs.n[4], cs.rows[4], cs.H[4], p, qc, dmask, gid); break;
-        case 5: pq = contact_rows_spmv<5, V>(cs.n[5], cs.rows[5], cs.H[5], p, qc, dmask, gid); break;
-        default: pq = contact_rows_spmv<6, V>(cs.n[6], cs.rows[6], cs.H[6], p, qc, dmask, gid); break;
+        case 0: pq = contact_rows_spmv<0, V>(cs.n[0], cs.rows[0], cs.H[0], cs.Hf[0], cs.c64[0], p, qc, dmask, gid); break;
+        case 1: pq = contact_rows_spmv<1, V>(cs.n[1], cs.rows[1], cs.H[1], cs.Hf[1], cs.c64[1], p, qc, dmask, gid); break;
+        case 2: pq = contact_rows_spmv<2, V>(cs.n[2], cs.rows[2], cs.H[2], cs.Hf[2], cs.c64[2], p, qc, dmask, gid); break;
+        case 3: pq = contact_rows_spmv<3, V>(cs.n[3], cs.rows[3], cs.H[3], cs.Hf[3], cs.c64[3], p, qc, dmask, gid); break;
+        case 4: pq = contact_rows_spmv<4, V>(cs.n[4], cs.rows[4], cs.H[4], cs.Hf[4], cs.c64[4], p, qc, dmask, gid); break;
+        case 5: pq = contact_rows_spmv<5, V>(cs.n[5], cs.rows[5], cs.H[5], cs.Hf[5], cs.c64[5], p, qc, dmask, gid); break;
+        default: pq = contact_rows_spmv<6, V>(cs.n[6], cs.rows[6], cs.H[6], cs.Hf[6], cs.c64[6], p, qc, dmask, gid); break;
         }
     }
     if (part) {
@@ -1129,10 +1154,17 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     if (!c) return FIPC_OK;
     if (c->s) (void)hipStreamSynchronize(c->s);
     if (c->s2) (void)hipStreamSynchronize(c->s2);
+    // pending timing events are read while both streams still exist
+    if (c->s) (void)fipc_tm_fold(c);
+    for (auto& t : c->tm) {  // events left by a failed fold go back to the pool
+        for (auto& pr : t.pending) { c->event_pool.push_back(pr.first); c->event_pool.push_back(pr.second); }
+        t.pending.clear();
+    }
     if (c->pin) (void)hipHostFree(c->pin);
     if (c->ev_rhs) (void)hipEventDestroy(c->ev_rhs);
     if (c->ev_contact) (void)hipEventDestroy(c->ev_contact);
     if (c->s2) (void)hipStreamDestroy(c->s2);
+    c->s2 = nullptr;
     if (c->ws_done) (void)hipEventDestroy(c->ws_done);
     release(c->tets); release(c->bp); release(c->be); release(c->bt); release(c->dids);
     release(c->IB); release(c->vol); release(c->la); release(c->mu); release(c->m); release(c->dmask);
@@ -1142,8 +1174,8 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->brow); release(c->tetH); release(c->tetg); release(c->bcptr); release(c->cpos); release(c->vcptr);
     release(c->lptr); release(c->lent);
     release(c->vcent);
-    for (int t = 0; t < NTYPES; t++) { release(c->rows[t]); release(c->cH[t]); }
-    release(c->keys_pp); release(c->keys_pe); release(c->keys_tmp); release(c->counters); release(c->cub_tmp); release(c->cub_tmp2);
+    for (int t = 0; t < NTYPES; t++) { release(c->rows[t]); release(c->cH[t]); release(c->cHf[t]); release(c->c64[t]); }
+    release(c->keys_pp); release(c->keys_pe); release(c->keys_tmp); release(c->counters); release(c->cub_tmp); release(c->cub_tmp2); release(c->rows_tmp);
     release(c->nuniq); release(c->cdiag); release(c->Minv); release(c->r); release(c->z); release(c->pp); release(c->q); release(c->qc);
     release(c->Minvf); release(c->rf); release(c->zf); release(c->pf); release(c->qf);
     release(c->trow); release(c->tval);
@@ -1157,7 +1189,6 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->ccd_pairs[0]); release(c->ccd_pairs[1]);
     release(c->dprev); release(c->wsX); release(c->ckey); release(c->free_ids); release(c->cap_ids);
     release(c->cap_X); release(c->cap_T); release(c->near_ids); release(c->nnear);
-    if (c->s) (void)fipc_tm_fold(c);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     if (c->s) (void)hipStreamDestroy(c->s);
     delete c;
@@ -1456,6 +1487,13 @@ static int launch_contacts(fipc_ctx* c, int accumulate, hipStream_t st)
         cb.rows[t] = c->rows[t].p;
         cb.H[t] = c->cH[t].p;
         cb.n[t] = c->count[t];
+        const int N = kind_dofs(t);
+        if (c->precision == 1) {
+            STCHK(ensure(c->cHf[t], (size_t)std::max(c->count[t], 1) * N * N));
+            STCHK(ensure(c->c64[t], (size_t)std::max(c->count[t], 1)));
+        }
+        cb.Hf[t] = c->precision == 1 ? c->cHf[t].p : nullptr;
+        cb.c64[t] = c->precision == 1 ? c->c64[t].p : nullptr;
     }
     int nb = contact_blocks(cb);
     if (!nb) return FIPC_OK;
@@ -1506,13 +1544,15 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
 
 // ---- SpMV ----
 // contact blocks follow the BSR blocks; returns the total grid size
-static int spmv_contacts(fipc_ctx* c, ContactSet& cs)
+static int spmv_contacts(fipc_ctx* c, ContactSet& cs, bool fp32 = false)
 {
     int nb = 0;
     for (int t = 0; t < NTYPES; t++) {
         int K = t == 0 ? 2 : (t == 1 ? 3 : 4);
         cs.rows[t] = c->rows[t].p;
         cs.H[t] = c->cH[t].p;
+        cs.Hf[t] = fp32 ? c->cHf[t].p : nullptr;
+        cs.c64[t] = fp32 ? c->c64[t].p : nullptr;
         cs.n[t] = c->count[t];
         cs.boff[t] = nb;
         nb += nblk((long)c->count[t] * K);
@@ -1526,7 +1566,8 @@ template <class V>
 static int spmv(fipc_ctx* c, int prec, const V* p, V* q, double* qc, double* part, const int* state, int it)
 {
     ContactSet cs;
-    int nbc = spmv_contacts(c, cs);
+    // the FP32 iterations (FP32 vectors) read the FP32 contact copies
+    int nbc = spmv_contacts(c, cs, prec == 1 && sizeof(V) == 4);
     int nbb = spmv_grid(c->nv);
     // products inside the PCG iterations and the residual products (initial
     // guess, FP64 refinement) are timed as separate classes
@@ -1688,9 +1729,15 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
         if (prec != 1 || state[0] == -1) { converged = true; break; }
         // FP32 iterations converged: the next round checks the FP64 residual
     }
-    // refinement rounds exhausted after a converged FP32 round: accept it
-    // (relres then reports the recursive residual of that round)
-    if (!converged && state[0] != 0x7fffffff) converged = true;
+    // refinement rounds exhausted after a converged FP32 round: the FP64
+    // residual of its result decides (one more product, no iterations); the
+    // solve is reported as not converged when it misses the tolerance
+    if (!converged && state[0] != 0x7fffffff) {
+        int chk[2] = {0, 0};
+        STCHK((pcg_round<float, float>(c, prec, c->Minvf.p, c->rf.p, c->zf.p, c->pf.p, c->qf.p,
+                                       first_round + rounds, 0, tol2, 0, chk, false, &init_only)));
+        converged = chk[0] == -1;
+    }
     // a solve that ended on a restart whose FP64 residual met the tolerance
     // leaves that restart for a continuation
     c->restart_ready = prec == 1 && init_only;
@@ -1983,7 +2030,10 @@ extern "C" int fipc_batch_constraint(int type, int n, const double* x, const dou
     HIPCHK(hipMemcpy(dx0, x0 ? x0 : x, npts * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(drows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
     ContactBatch cb;
-    for (int t = 0; t < NTYPES; t++) { cb.rows[t] = drows; cb.H[t] = dH; cb.n[t] = t == type ? n : 0; }
+    for (int t = 0; t < NTYPES; t++) {
+        cb.rows[t] = drows; cb.H[t] = dH; cb.n[t] = t == type ? n : 0;
+        cb.Hf[t] = nullptr; cb.c64[t] = nullptr;
+    }
     int nb = contact_blocks(cb);
     k_contact<<<nb, TPB>>>(cb, dx, dx0, dHat2, kappa, nullptr, nullptr, dE, dg, project, 0, nullptr, 20);
     HIPCHK(hipGetLastError());

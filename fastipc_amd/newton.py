@@ -16,7 +16,10 @@ Differences, all documented: the linear solve is a block-Jacobi PCG instead
 of CHOLMOD; Dirichlet vertices whose prescribed position moves between steps
 reach it through a prescribed increment in the solve (static Dirichlet
 vertices behave exactly as in the reference); safety caps bound the Newton and
-halving loops, which the reference leaves unbounded.
+halving loops, which the reference leaves unbounded, and a step whose line
+search no longer moves x (alpha max|p| below 1e-3 of the Newton tolerance)
+ends there and is reported as stalled, where the reference would loop
+forever.
 
 Inexact search directions (``pcg_tol_search``, off by default): the solves
 whose direction is applied stop at the looser relative tolerance
@@ -51,6 +54,13 @@ class StepStats:
     energy: float = float("nan")
     converged: bool = True
     warm_accepted: bool = False
+    # solves that ended without meeting the PCG tolerance (their directions
+    # are applied by the line search but never decide convergence)
+    unconverged_solves: int = 0
+    relres: list = field(default_factory=list)
+    # the line search stopped making progress: the applied step alpha max|p|
+    # fell below 1e-3 of the Newton tolerance while the test still failed
+    stalled: bool = False
 
 
 class NewtonIPC:
@@ -67,6 +77,7 @@ class NewtonIPC:
         self.pcg_tol_search = pcg_tol_search
         self.pcg_maxit = pcg_maxit
         self.last_pcg_iters = 0
+        self.last_solve_converged = True
         self.max_newton = max_newton
         self.max_halvings = max_halvings
         self.inversion = inversion
@@ -167,6 +178,19 @@ class NewtonIPC:
         self._E = None
         return self.counts
 
+    def search_stats(self, on: bool = True):
+        """Count the broad-phase candidates of the constraint searches from
+        now on (off by default: the counting slows the queries)."""
+        _lib.call("fipc_search_stats", self.ctx, int(on))
+
+    def search_counts(self):
+        """Broad-phase candidates (point-triangle, edge-edge) of the last
+        constraint search: pairs whose boxes overlap within dHat (-1 unless
+        search_stats was on for that search)."""
+        cand = np.zeros(2, np.int32)
+        _lib.call("fipc_search_counts", self.ctx, _lib.iptr(cand))
+        return [int(c) for c in cand]
+
     def constraints(self, sort: bool = True):
         """Host copy of the seven row sets, in np.unique order when ``sort``."""
         out = {}
@@ -196,6 +220,7 @@ class NewtonIPC:
         if code not in (0, 5):
             raise _lib.FipcError("fipc_solve", code)
         self.last_pcg_iters = it.value
+        self.last_solve_converged = code == 0
         return it.value, rel.value
 
     def refine_solve(self, tol: float | None = None):
@@ -208,6 +233,7 @@ class NewtonIPC:
         code = self.so.fipc_solve_refine(self.ctx, tol, maxit, C.byref(it), C.byref(rel))
         if code not in (0, 5):
             raise _lib.FipcError("fipc_solve_refine", code)
+        self.last_solve_converged = code == 0
         return it.value, rel.value
 
     def output_residual(self) -> float:
@@ -277,7 +303,7 @@ class NewtonIPC:
                 with T("solve"):
                     it, rel = self.solve_system(f * dt, self.pcg_tol_search)
                 res = self.output_residual()
-                if self.pcg_tol_search is not None and res < self.setup.newton_tol:
+                if self.pcg_tol_search is not None and res < self.setup.newton_tol and self.last_solve_converged:
                     # the direction that decides convergence is solved to pcg_tol
                     with T("solve"):
                         it2, rel = self.refine_solve()
@@ -285,7 +311,13 @@ class NewtonIPC:
                     res = self.output_residual()
                 st.pcg_iters.append(it)
                 st.residuals.append(res)
-                if res < self.setup.newton_tol:
+                st.relres.append(rel)
+                if not self.last_solve_converged:
+                    # a direction short of the PCG tolerance underestimates
+                    # max |p| (CG iterates grow from zero): it is applied,
+                    # never trusted for the convergence test
+                    st.unconverged_solves += 1
+                elif res < self.setup.newton_tol:
                     break
                 if st.newton_iters >= self.max_newton:
                     st.converged = False
@@ -293,6 +325,16 @@ class NewtonIPC:
                 with T("line_search"):
                     alpha = self._line_search()
                     st.alphas.append(alpha)
+                if alpha * res < 1e-3 * self.setup.newton_tol:
+                    # The reference would iterate forever here: its energy is
+                    # discontinuous where an edge pair's closest-feature type
+                    # changes and np.unique had merged the point-edge row it
+                    # emitted with one from the point-triangle search
+                    # (NEWTON_IPC.py:235-357), so E > E0 for every step the
+                    # halving tries.  The step ends at the last iterate.
+                    st.stalled = True
+                    st.converged = False
+                    break
             if capture is not None:
                 capture.end(self)
             self.compute_v()

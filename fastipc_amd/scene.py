@@ -169,7 +169,10 @@ class Scene:
     dirichlet_rows_fn: Optional[Callable[[float], np.ndarray]] = None
     boundary: tuple = None        # (points, edges, triangles)
     dhat_rel: float = 1e-3        # dhat = dhat_rel * bbox diagonal
-    tol_rel: float = 1e-2         # Newton stop: max|p| < tol_rel * dt * bbox diagonal
+    tol_rel: float = 1e-2         # Newton stop: max|p| < tol_rel * dt * tol_len
+    # length of the Newton tolerance: the bbox diagonal when None; 1.0 gives
+    # the reference's absolute test max|p| < 1e-2 dt (NEWTON_IPC.py:975)
+    tol_len: Optional[float] = None
     eta: float = 0.2              # CCD gap fraction (NEWTON_IPC.py:405, :439)
     seed: int = 0
 
@@ -195,7 +198,7 @@ class Scene:
 
     @property
     def newton_tol(self):
-        return self.tol_rel * self.dt * self.bbox_diag
+        return self.tol_rel * self.dt * (self.bbox_diag if self.tol_len is None else self.tol_len)
 
     def dirichlet(self, t: float) -> np.ndarray:
         """Prescribed positions (V,3) at time ``t``; only masked rows matter."""
@@ -297,7 +300,12 @@ def cube_grid(n: int = 16, grid: int = 3, seed: int = 0, steps: int = 100) -> Sc
                    ([gap, gap, -th], [L - gap, H, 0.0]), ([gap, gap, L], [L - gap, H, L + th])]:
         gx, gt = box(lo, hi)
         b.add(gx, gt, fixed=True)
-    return b.build(name=f"cube_grid_{grid}x{n}", dt=0.01, steps=steps, seed=seed)
+    # the reference's absolute Newton tolerance 1e-2 dt: scaled by this
+    # scene's 18 m diagonal it would exceed a step's free-fall increment
+    # dt^2 g = 9.8e-4, the test would pass before the first direction is
+    # applied (NEWTON_IPC.py:975 breaks before the line search) and the
+    # cubes would never move
+    return b.build(name=f"cube_grid_{grid}x{n}", dt=0.01, steps=steps, seed=seed, tol_len=1.0)
 
 
 def cube_column(n: int = 8, count: int = 200, seed: int = 0, width: float = 5.6,
@@ -328,7 +336,9 @@ def cube_column(n: int = 8, count: int = 200, seed: int = 0, width: float = 5.6,
         b.add(xc + p, tc)
     gx, gt = box([-0.5, -0.2, -0.5], [width + 0.5, 0.0, width + 0.5])
     b.add(gx, gt, fixed=True)
-    return b.build(name=f"cube_column_{count}x{n}", dt=0.01, steps=steps, seed=seed)
+    # the reference's absolute Newton tolerance, as for cube_grid (this
+    # scene's diagonal is 24 m)
+    return b.build(name=f"cube_column_{count}x{n}", dt=0.01, steps=steps, seed=seed, tol_len=1.0)
 
 
 def compressed_slab(nx: int = 50, ny: int = 80, nz: int = 50, steps: int = 200,

@@ -163,8 +163,19 @@ int fipc_begin_step_rows(fipc_ctx* c, int n, const int* rows, const double* targ
 int fipc_end_step(fipc_ctx* c);
 /* find_constraints (NEWTON_IPC.py:337-357): counts[7] after deduplication */
 int fipc_find_constraints(fipc_ctx* c, int* counts);
-/* rows of one type in lexicographic order for PP/PE (np.unique order) and in
- * device order for the others (already unique); rows_out: counts[type]*width */
+/* broad-phase candidates of the last fipc_find_constraints: cand[0] the
+ * (boundary point, boundary triangle) pairs, cand[1] the (edge, edge) pairs
+ * i < j without a shared vertex, whose boxes overlap within dHat
+ * (point_triangle_ccd_broadphase / edge_edge_ccd_broadphase, ipc.py:332-354)
+ * and so go on to the closest-feature distance test (NEWTON_IPC.py:181-312).
+ * Counted only while fipc_search_stats(c, 1) is on (the counting costs the
+ * queries time); -1 otherwise.  No counterpart in the reference. */
+int fipc_search_stats(fipc_ctx* c, int on);
+int fipc_search_counts(fipc_ctx* c, int* cand);
+/* rows of one type, in lexicographic order (np.unique order,
+ * NEWTON_IPC.py:348-356): PP/PE are radix-sorted and deduplicated on the
+ * device, the four-index types (unique by construction) merge-sorted;
+ * rows_out: counts[type]*width */
 int fipc_get_constraints(fipc_ctx* c, int type, int* rows_out);
 /* output_current_minimal_distance (NEWTON_IPC.py:815-839) */
 int fipc_min_distance(fipc_ctx* c, double* d2min);
@@ -216,8 +227,10 @@ int fipc_spmv(fipc_ctx* c, const double* x, double* y);
  * 0 SpMV inside the PCG iterations (contact + BSR rows), 1 per-tet elastic
  * pass, 2 CCD query, 3 constraint / collision query, 4 energy, 5 contact
  * assembly, 6 PCG vector updates, 7 grid build (phase time on the stream),
- * 8 assembly gathers, 9 residual SpMVs (initial guess, FP64 refinement).
- * Writes the first n (<= 10) totals (ms) and launch counts. */
+ * 8 assembly gathers, 9 residual SpMVs (initial guess, FP64 refinement),
+ * 10 CCD narrow phase (the root finding over the candidate pairs; class 2 is
+ * then the candidate queries).  Writes the first n (<= 11) totals (ms) and
+ * launch counts. */
 int fipc_enable_timing(fipc_ctx* c, int on);
 int fipc_kernel_times(fipc_ctx* c, int n, double* ms, long* launches);
 int fipc_sync(fipc_ctx* c);

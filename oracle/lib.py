@@ -45,6 +45,7 @@ def _load():
         "orc_quad_root": (_D, [_D] * 4), "orc_cubic_root": (_D, [_D] * 5),
         "orc_tet_inversion_alpha": (_D, [_P] * 8 + [_D]),
         "orc_pt_ccd": (_D, [_P] * 8 + [_D, _D]), "orc_ee_ccd": (_D, [_P] * 8 + [_D, _D]),
+        "orc_ccd_one": (_D, [C.c_int, C.c_int] + [_P] * 8 + [_D, _D]),
         "orc_seg_tri": (C.c_int, [_P] * 5),
         "orc_find_constraints": (None, [_P, _P, C.c_int, _IP, C.c_int, _IP, C.c_int, _IP, _D, _D,
                                         C.POINTER(_IP), _LP, _LP]),
@@ -188,6 +189,13 @@ def ee_ccd(a0, a1, b0, b1, da0, da1, db0, db1, eta, dist2):
 
 def seg_tri(P, Q, A, B, Cc):
     return lib().orc_seg_tri(*[dptr(f64(v)) for v in (P, Q, A, B, Cc)])
+
+
+def ccd_one(kind, fused, pts, disp, eta, dist2):
+    """One rounding of the CCD alone (kind 0 point-triangle, 1 edge-edge):
+    fused multiply-adds or strict IEEE; orc_pt_ccd / orc_ee_ccd return the
+    smaller of the two."""
+    return lib().orc_ccd_one(int(kind), int(fused), *[dptr(f64(v)) for v in list(pts) + list(disp)], eta, dist2)
 
 
 def tet_inversion_alpha(xs, ps, eta=0.2):

@@ -217,6 +217,7 @@ class OracleIPC:
         self.compute_xn_and_xTilde(f * self.dt)
         it = 0
         alphas, residuals = [], []
+        stalled = False
         while True:
             it += 1
             self.find_constraints()
@@ -250,6 +251,9 @@ class OracleIPC:
                 self.apply_sol(alpha)
                 n += 1
             alphas.append(alpha)
+            if alpha * res < 1e-3 * self.tol:
+                stalled = True  # the product's stall exit, same criterion
+                break
         self.compute_v()
         self.frame += 1
-        return {"newton_iters": it, "alphas": alphas, "residuals": residuals}
+        return {"newton_iters": it, "alphas": alphas, "residuals": residuals, "stalled": stalled}

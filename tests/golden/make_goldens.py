@@ -10,6 +10,8 @@
   definitions; derivatives come from torch, independent of the oracle's jets.
 * reference_dense.npz -- solve_N, svd_2 and project_pd_{2,4} of the same
   native library, on seeded SPD / indefinite / diagonal inputs;
+* reference_ccd_scenes.npz (--scene-ccd) -- the reference's CCD step bounds
+  on the candidate pairs of stacked-cube and slab states (scene_ccd_set);
 * reference.npz -- outputs of the reference's own native library
   (common/math/wrapper/wrapper.cpp + EVCTCD, built by oracle/build_ref.sh into
   oracle/_ref/libref.so): project_pd_{3,6,9,12}, the cubic root and the
@@ -232,6 +234,101 @@ def dense_set(rng):
         out[f"pd{n}_out"] = np.array([O.ref_project_pd(a) for a in A])
     return out
 
+
+def _ccd_candidates(o, p, eta=0.2):
+    """All (point, triangle) and (edge i < j) pairs of an oracle state that
+    reach the CCD narrow phase for direction p, by brute force: index
+    filter, the swept boxes overlapping within dHat (ipc.py:339-345,
+    :357-363) and the lower bound (1 - eta) d / maxDisp below 1
+    (wrapper.cpp:254-328).  Returns (kind, points, displacements, d^2)."""
+    from oracle import lib as O
+    L = O.lib()
+    x, dh = o.x, o.dHat
+    bp, be, bt = o.bp, o.be, o.bt
+    out = []
+    lo_p, hi_p = np.minimum(x[bp], x[bp] + p[bp]), np.maximum(x[bp], x[bp] + p[bp])
+    xt, pt = x[bt], p[bt]
+    lo_t, hi_t = np.minimum(xt, xt + pt).min(1), np.maximum(xt, xt + pt).max(1)
+    for i in range(len(bp)):
+        ok = np.all(lo_p[i] < hi_t + dh, axis=1) & np.all(lo_t - dh < hi_p[i], axis=1) & ~np.any(bt == bp[i], axis=1)
+        for j in np.flatnonzero(ok):
+            ids = [bp[i], *bt[j]]
+            P, D = x[ids], p[ids]
+            md = np.sqrt(D[0] @ D[0]) + np.sqrt(max(D[1] @ D[1], D[2] @ D[2], D[3] @ D[3]))
+            if md == 0:
+                continue
+            d2 = L.orc_PT_dist2(*[O.dptr(np.ascontiguousarray(v)) for v in P], O.pt_type(*P))
+            if (1 - eta) * np.sqrt(d2) / md < 1:
+                out.append((0, P, D, d2))
+    xe, pe = x[be], p[be]
+    lo_e, hi_e = np.minimum(xe, xe + pe).min(1), np.maximum(xe, xe + pe).max(1)
+    for i in range(len(be)):
+        a0, a1 = be[i]
+        ok = np.all(lo_e[i] < hi_e + dh, axis=1) & np.all(lo_e - dh < hi_e[i], axis=1)
+        ok[:i + 1] = False
+        ok &= ~(np.isin(be[:, 0], [a0, a1]) | np.isin(be[:, 1], [a0, a1]))
+        for j in np.flatnonzero(ok):
+            ids = [a0, a1, *be[j]]
+            P, D = x[ids], p[ids]
+            md = np.sqrt(max(D[0] @ D[0], D[1] @ D[1])) + np.sqrt(max(D[2] @ D[2], D[3] @ D[3]))
+            if md == 0:
+                continue
+            d2 = L.orc_EE_dist2(*[O.dptr(np.ascontiguousarray(v)) for v in P], O.ee_type(*P))
+            if (1 - eta) * np.sqrt(d2) / md < 1:
+                out.append((1, P, D, d2))
+    return out
+
+
+def scene_ccd_set(rng):
+    """Golden step bounds of the reference's compiled CCD for the pairs real
+    scene states send to the narrow phase: the stacked cubes (two 10^3
+    cubes, steps 8 and 20: before contact the cubes fall rigidly on the
+    lattice, so many edge pairs are exactly parallel) and the 10x16x10 slab
+    between plates (step 15), each with the oracle's Newton direction scaled
+    1, 30 and 300.  Kept: every pair whose strict-IEEE and fused-multiply-add
+    evaluations differ by more than 1e-9, or where either differs from the
+    reference by more than that, and a seeded sample of 600 others per state
+    and scale."""
+    from fastipc_amd import scene as S
+    from oracle import lib as O
+    from oracle.newton import OracleIPC
+    L = O.lib()
+    import ctypes as C
+    rows = {k: [] for k in ("kind", "pts", "disp", "d2", "toc", "state")}
+    for label, sc, steps in (("cubes", S.two_cubes(10), (8, 20)), ("slab", S.compressed_slab(10, 16, 10), (15,))):
+        o = OracleIPC(sc)
+        for k in range(max(steps) + 1):
+            if k in steps:
+                o.compute_xn_and_xTilde(o.frame * o.dt)
+                o.find_constraints()
+                o.assemble()
+                o.solve_system(o.frame * o.dt)
+                p0 = o.p.copy()
+                for si, scale in enumerate((1.0, 30.0, 300.0)):
+                    cand = _ccd_candidates(o, p0 * scale)
+                    ref = np.array([(O.ref_pt_ccd if kk == 0 else O.ref_ee_ccd)(*P, *D, 0.2, d2) for kk, P, D, d2 in cand])
+                    one = np.array([[O.ccd_one(kk, f, P, D, 0.2, d2) for f in (0, 1)] for kk, P, D, d2 in cand])
+                    keep = np.flatnonzero((np.abs(ref[:, None] - one) > 1e-9).any(axis=1) | (np.abs(one[:, 0] - one[:, 1]) > 1e-9))
+                    rest = np.setdiff1d(np.arange(len(cand)), keep)
+                    keep = np.concatenate([keep, rng.choice(rest, min(600, len(rest)), replace=False)])
+                    for i in np.sort(keep):
+                        kk, P, D, d2 = cand[i]
+                        rows["kind"].append(kk)
+                        rows["pts"].append(P)
+                        rows["disp"].append(D)
+                        rows["d2"].append(d2)
+                        rows["toc"].append(ref[i])
+                        rows["state"].append(f"{label}:{k}:{scale:g}")
+                    print(f"{label} step {k} scale {scale:g}: {len(cand)} pairs, kept {len(keep)}", flush=True)
+            o.step()
+    return {"kind": np.array(rows["kind"], np.int32), "pts": np.array(rows["pts"]), "disp": np.array(rows["disp"]),
+            "d2": np.array(rows["d2"]), "toc": np.array(rows["toc"]), "state": np.array(rows["state"])}
+
+
+if __name__ == "__main__" and "--scene-ccd" in sys.argv:
+    np.savez_compressed(os.path.join(HERE, "reference_ccd_scenes.npz"), **scene_ccd_set(np.random.default_rng(31)))
+    print("wrote reference_ccd_scenes.npz")
+    sys.exit(0)
 
 if __name__ == "__main__":
     rng = np.random.default_rng(20261202)

@@ -214,3 +214,52 @@ def test_energy_reuse_matches_full_evaluation():
     e3 = g.compute_energy(parts=True)
     assert e2[0] == e3[0] and np.array_equal(e2[1], e3[1])
     assert e1[1][1] == e3[1][1] and e1[1][2] == e3[1][2] and e1[0] != e3[0]
+
+
+def test_fp32_per_step_error_against_fp64_on_cube_column():
+    """The FP32-Hessian form on the 200-cube column (configs[2], 343k tets)
+    against FP64, step by step from the same state (the FP64 run's): after
+    each step the FP32 step's positions, its incremental potential and the
+    norm of its gradient (both evaluated by the FP64 oracle at the end
+    state) are compared with the FP64 step's.  Both steps stop anywhere
+    within the Newton tolerance of the same minimiser, so positions differ by
+    at most about twice the tolerance; the printed table is the per-step
+    error the north star asks to state."""
+    sc = S.config(2)
+    g64 = NewtonIPC(sc)
+    g32 = NewtonIPC(sc, precision=1)
+    o = OracleIPC(sc)
+    tol = sc.newton_tol
+    rows = []
+    for f in range(40):
+        x0, v0 = g64.x, g64.v
+        g32.set_state(x0, v0)
+        g32.frame = g64.frame
+        s64 = g64.step()
+        s32 = g32.step()
+        assert (s64.converged or s64.stalled) and (s32.converged or s32.stalled)
+        assert g32.check_collision() == 0
+        if f < 18 or f % 4:
+            continue  # contact begins near step 15; the oracle evaluates every fourth step after it
+        x64, x32 = g64.x, g32.x
+        out = []
+        for x in (x64, x32):
+            o.x, o.v = np.ascontiguousarray(x0), v0.copy()
+            o.compute_xn_and_xTilde(f * sc.dt)  # x_n and xTilde of this step
+            o.x = np.ascontiguousarray(x)
+            o.find_constraints()
+            E = o.compute_energy()
+            o.assemble()
+            D = np.repeat(o.dmask, 3)
+            gvec = o.rhs.copy()
+            gvec[D] = 0.0
+            out.append((E, np.linalg.norm(gvec)))
+        dx = np.abs(x32 - x64).max()
+        rows.append((f, s64.newton_iters, s32.newton_iters, dx / tol, abs(out[1][0] - out[0][0]) / abs(out[0][0]),
+                     out[0][1], out[1][1], sum(o.constraints()[k].shape[0] for k in ("PT", "EE"))))
+        assert dx <= 2.5 * tol, (f, dx, tol)
+        assert abs(out[1][0] - out[0][0]) <= 1e-6 * abs(out[0][0])
+    print("step  newton64 newton32  max|x32-x64|/tol  |E32-E64|/|E64|  |g64|  |g32|  PT+EE rows")
+    for r in rows:
+        print(f"{r[0]:4d} {r[1]:8d} {r[2]:8d} {r[3]:16.3e} {r[4]:16.3e} {r[5]:.3e} {r[6]:.3e} {r[7]}")
+    assert rows and rows[-1][7] > 0

@@ -131,6 +131,29 @@ def test_ccd_matches_oracle(kind):
     assert np.all(toc <= ref + 1e-12)
 
 
+def test_ccd_scene_pairs_match_oracle_and_reference():
+    """The narrow phase on the pairs real scene states produce (golden
+    reference_ccd_scenes.npz, the reference's compiled CCD): equal to the
+    oracle (both the smaller of the strict and fused-multiply-add
+    evaluations), never later than the reference except its proven false
+    hits (test_oracle_pins.test_ccd_scene_pairs_vs_reference)."""
+    import os
+    F = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_ccd_scenes.npz"))
+    for kind in (0, 1):
+        m = F["kind"] == kind
+        pts, disp, d2, ref = F["pts"][m], F["disp"][m], np.ascontiguousarray(F["d2"][m]), F["toc"][m]
+        n = len(ref)
+        toc = np.zeros(n)
+        _lib.call("fipc_batch_ccd", kind, n, _lib.dptr(np.ascontiguousarray(pts.reshape(n, 12))),
+                  _lib.dptr(np.ascontiguousarray(disp.reshape(n, 12))), 0.2, _lib.dptr(d2), _lib.dptr(toc))
+        orc = np.array([(O.pt_ccd if kind == 0 else O.ee_ccd)(*pts[i], *disp[i], 0.2, d2[i]) for i in range(n)])
+        assert np.abs(toc - orc).max() <= 1e-12, int(np.sum(np.abs(toc - orc) > 1e-12))
+        later = np.flatnonzero(toc > ref + 1e-9)
+        assert np.all(orc[later] > ref[later] + 1e-9) and len(later) <= 2
+        print(f"kind {kind}: {n} pairs, {int(np.sum(np.abs(toc - ref) > 1e-6))} outside 1e-6 of the reference, "
+              f"{len(later)} later (false hits of the reference)")
+
+
 def test_inversion_bound_matches_oracle():
     """The inversion part of the line-search filter (NEWTON_IPC.py:443-453)
     equals the oracle's minimum over every tet, for displacements that invert

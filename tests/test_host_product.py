@@ -344,3 +344,15 @@ def test_neohookean_tifunc_dropins():
             fdg = (_lib.scalar("elasticity_gradient", *(s + e), 4e5, 1e5)
                    - _lib.scalar("elasticity_gradient", *(s - e), 4e5, 1e5)) / (2 * eps)
             assert np.abs(Hs[:, a] - fdg).max() <= 1e-6 * np.abs(Hs).max()
+
+
+def test_ccd_dropins_scene_pairs_equal_oracle():
+    """The host build of the device CCD equals the oracle bit for bit on the
+    scene pairs of reference_ccd_scenes.npz (both take the smaller of the
+    strict and the fused-multiply-add evaluation)."""
+    F = np.load(os.path.join(ROOT, "tests", "golden", "reference_ccd_scenes.npz"))
+    for i in range(len(F["kind"])):
+        k, P, D, d2 = int(F["kind"][i]), F["pts"][i], F["disp"][i], float(F["d2"][i])
+        host = _lib.scalar("point_triangle_ccd" if k == 0 else "edge_edge_ccd", *P.ravel(), *D.ravel(), 0.2, d2)[0]
+        orc = (O.pt_ccd if k == 0 else O.ee_ccd)(*P, *D, 0.2, d2)
+        assert host == orc, (i, host, orc)

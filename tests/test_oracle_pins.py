@@ -401,3 +401,93 @@ def test_tet_gradient_hessian_vs_autodiff():
                      la, mu, dt, O.dptr(g12), O.dptr(H12))
         assert rel(g12, G) < 1e-9
         assert rel(H12.reshape(12, 12), H) < 1e-7
+
+
+def test_reference_line_search_stall():
+    """Why the reference's Newton loop can stall: at the stalled iterate of
+    a cube-grid step (tests/golden/stall_local.npz, made by
+    make_stall_fixture.py) the edge pair a = (1, 3), b = (4, 5) is closest
+    at b's end point 5 (EE type 7), so the edge-edge search emits the
+    point-edge row (5, 1, 3) (NEWTON_IPC.py:235-312) -- the same tuple the
+    point-triangle search emits for point 5 and triangle (1, 3, 0), and
+    np.unique (:348-356) keeps one.  A step of 1e-3 along the Newton
+    direction moves the closest points inside both edges (type 8): the
+    edge-edge row (1, 3, 4, 5) appears while the point-edge row stays (the
+    triangle still yields it), so the barrier energy jumps up by a whole
+    barrier term at an unchanged distance.  E > E0 then holds for every
+    step the line search tries and alpha is halved without end."""
+    F = np.load(os.path.join(HERE, "golden", "stall_local.npz"))
+    x, p, x0 = F["x"], F["p"], F["x0"]
+    tris, edges = F["tris"], F["edges"]
+    dhat, kappa, inv_dx = float(F["dhat"]), float(F["kappa"]), float(F["inv_dx"])
+    bp = np.arange(len(x), dtype=np.int32)
+    L = O.lib()
+    import ctypes as C
+
+    def search(xx):
+        cap = 64
+        bufs = [np.zeros((cap, w), np.int32) for w in O.WIDTH]
+        arr = (C.POINTER(C.c_int) * 7)(*[O.iptr(b) for b in bufs])
+        caps = np.full(7, cap, np.int64)
+        cnt = np.zeros(7, np.int64)
+        xx = np.ascontiguousarray(xx)
+        L.orc_find_constraints(O.dptr(xx), O.dptr(np.ascontiguousarray(x0)), len(bp), O.iptr(bp), len(edges),
+                               O.iptr(np.ascontiguousarray(edges)), len(tris), O.iptr(np.ascontiguousarray(tris)),
+                               dhat, inv_dx, arr, caps.ctypes.data_as(C.POINTER(C.c_int64)),
+                               cnt.ctypes.data_as(C.POINTER(C.c_int64)))
+        rows = {O.KINDS[t]: set(map(tuple, np.unique(bufs[t][:cnt[t]], axis=0))) for t in range(7)}
+        E = sum(O.constraint_E(t, xx[list(r)], x0[list(r)], dhat * dhat, kappa)
+                for t in range(7) for r in rows[O.KINDS[t]])
+        return rows, E
+
+    r0, E0 = search(x)
+    r1, E1 = search(x + 1e-3 * p)
+    pts = [np.ascontiguousarray(v) for v in x]
+    assert O.ee_type(pts[1], pts[3], pts[4], pts[5]) == 7
+    q = [np.ascontiguousarray(v) for v in x + 1e-3 * p]
+    assert O.ee_type(q[1], q[3], q[4], q[5]) == 8
+    assert (5, 1, 3) in r0["PE"] and (5, 1, 3) in r1["PE"]
+    assert (1, 3, 4, 5) not in r0["EE"] and (1, 3, 4, 5) in r1["EE"]
+    jump = O.constraint_E(3, x[[1, 3, 4, 5]], x0[[1, 3, 4, 5]], dhat * dhat, kappa)
+    assert jump > 0
+    # the energy of these rows rises by about one barrier term, far more
+    # than the first-order change along the direction could explain
+    assert E1 - E0 == pytest.approx(jump, rel=1e-2)
+
+
+def test_ccd_scene_pairs_vs_reference():
+    """The CCD step bounds of the pairs real scene states send to the narrow
+    phase, against the reference's own compiled CCD (golden
+    reference_ccd_scenes.npz): never later than the reference, except where
+    the reference's hit is false (brute force: no contact).  Lattice-
+    aligned edges falling rigidly (the stacked cubes before contact) make
+    coefficients that vanish in exact arithmetic; the strict-IEEE rounding
+    alone is then later than the reference on some pairs (what the
+    fused-multiply-add twin, and so the minimum of the two, corrects), and
+    the pairs outside 1e-6 are all of that kind and all earlier."""
+    F = np.load(os.path.join(HERE, "golden", "reference_ccd_scenes.npz"))
+    kind, P, D, d2, toc, state = F["kind"], F["pts"], F["disp"], F["d2"], F["toc"], F["state"]
+    orc = np.array([(O.pt_ccd if k == 0 else O.ee_ccd)(*P[i], *D[i], 0.2, d2[i]) for i, k in enumerate(kind)])
+    strict = np.array([O.ccd_one(k, 0, P[i], D[i], 0.2, d2[i]) for i, k in enumerate(kind)])
+    assert len(toc) > 4000 and (toc < 1).sum() > 300
+    later = np.flatnonzero(orc > toc + 1e-9)
+    L = O.lib()
+    for i in later:
+        # later than the reference only where the reference reports a
+        # contact that does not happen: the exact segment distance along
+        # the whole step never comes near eta * d (a false hit of its root
+        # finding, which our bound is not bound to repeat)
+        assert kind[i] == 1 and orc[i] == 1.0
+        eta_d = 0.2 * np.sqrt(d2[i])
+        dmin = min(L.orc_EE_dist2(*[O.dptr(np.ascontiguousarray(v)) for v in P[i] + t * D[i]],
+                                  O.ee_type(*(P[i] + t * D[i])))
+                   for t in np.linspace(0.0, 1.0, 2001))
+        assert np.sqrt(dmin) > 2 * eta_d, (i, np.sqrt(dmin), eta_d)
+    assert len(later) <= 2
+    outside = (np.abs(orc - toc) > 1e-6) & (orc <= toc)
+    print(f"{len(toc)} pairs: {int(outside.sum())} earlier than the reference by more than 1e-6, "
+          f"{len(later)} later (false hits of the reference); strict rounding alone later than the "
+          f"reference on {int(np.sum(strict > toc + 1e-9))}")
+    assert outside.sum() <= 50
+    assert set(state[outside]) <= {"cubes:8:30", "cubes:8:300"}
+    assert np.sum(strict > toc + 1e-9) > 0
