@@ -72,6 +72,7 @@ _SIGS = {
     "fipc_batch_constraint": [C.c_int, C.c_int, _P, _P, _D, _D, C.c_int, _P, _P, _P],
     "fipc_batch_tet": [C.c_int, _P, _P, _P, _P, _P, _P, _P, _P],
     "fipc_batch_ccd": [C.c_int, C.c_int, _P, _P, _D, _P, _P],
+    "fipc_batch_ccd_noalloc": [C.c_int, C.c_int, _P, _P, _D, _P, _P],
     "fipc_host_constraint": [C.c_int, _P, _P, _D, _D, C.c_int, _P, _P, _P],
     "fipc_host_tet": [_P, _P, _D, _D, _D, _P, _P, _P],
     "fipc_mlp_create": [C.POINTER(_VP), C.c_int, C.c_int, C.c_int, C.c_int, _U64],

@@ -31,7 +31,7 @@ extern "C" {
 #define FIPC_OK 0
 #define FIPC_ERR_ARG 1          /* invalid argument */
 #define FIPC_ERR_HIP 2          /* HIP runtime error */
-#define FIPC_ERR_CAPACITY 3     /* index range exceeds the packed-key limit (2^21 vertices) */
+#define FIPC_ERR_CAPACITY 3     /* a fixed limit exceeded: the packed-key limit (2^21 vertices), a batch maximum */
 #define FIPC_ERR_NONFINITE 4    /* NaN/Inf in a result */
 #define FIPC_ERR_NOT_CONVERGED 5 /* PCG hit its iteration limit */
 #define FIPC_ERR_STATE 6        /* call out of order (e.g. no mesh) */
@@ -122,6 +122,17 @@ int fipc_batch_tet(int n, const double* F, const double* IB, const double* scale
 /* CCD on the device: type 0 point-triangle, 1 edge-edge.  pts[n*12], disp[n*12],
  * dist2[n] -> toc[n] (wrapper.cpp:254-328 semantics) */
 int fipc_batch_ccd(int type, int n, const double* pts, const double* disp, double eta, const double* dist2, double* toc);
+/* fipc_batch_ccd without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * device copies of the inputs and of toc live in a fixed device array of
+ * FIPC_CCD_NOALLOC_MAX pairs per GPU (3.25 MiB, held until the process
+ * ends) instead of buffers allocated per call; larger n returns
+ * FIPC_ERR_CAPACITY.  Calls that overlap on one GPU share that array and
+ * race.  Same arguments and results as fipc_batch_ccd. */
+#define FIPC_CCD_NOALLOC_MAX 16384
+int fipc_batch_ccd_noalloc(int type, int n, const double* pts, const double* disp, double eta, const double* dist2,
+                           double* toc);
 
 /* host-side evaluation of the same per-constraint / per-tet math that the
  * kernels run (for CPU tests and debugging; no GPU needed).  Arguments as for
