@@ -92,9 +92,9 @@ def algorithmic_bytes(sim, precision):
         # not compulsory traffic
         "spmv": nu * (9 * sa + 4) + nl * 8 + 8 * (nv + 1) + 2 * 3 * nv * sa + contact,
         # per tet: 4 indices, D_m^-1, vol, lambda, mu, 10 contribution
-        # positions; per vertex x; the stored block values written once and
-        # the gradient
-        "elastic_assembly": nt * (16 + 72 + 24 + 40) + nv * 24 + nu * 9 * 8 + 3 * nv * 8,
+        # positions; per vertex x; the stored block values written once (FP32
+        # in FP32 mode) and the gradient
+        "elastic_assembly": nt * (16 + 72 + 24 + 40) + nv * 24 + nu * 9 * sa + 3 * nv * 8,
     }
 
 

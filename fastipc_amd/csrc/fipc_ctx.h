@@ -116,6 +116,7 @@ struct fipc_ctx {
     // by block, so the gather sums contiguous runs bcptr[b] .. bcptr[b + 1];
     // the gradient is gathered through vertex -> (corner, tet) lists
     DVec<double> tetH, tetg;
+    DVec<float> tetHf;     // the records in FP32 mode
     DVec<int> cpos, bcptr, vcptr, vcent;
     int precision = 0;     // 0 = FP64 Hessian, 1 = FP32 Hessian / SpMV
     // contacts
@@ -130,6 +131,19 @@ struct fipc_ctx {
     // 1e-2 dHat, whose FP64 Hessian those iterations keep using
     DVec<float> cHf[NTYPES];
     DVec<unsigned char> c64[NTYPES];
+    // Contact vertices: every (row, vertex) slot of the seven types, slot
+    // s = soff[t] + K_t r + a, keyed by its vertex and stably sorted, so the
+    // slots of a vertex are contiguous and in (type, row, vertex) order;
+    // cv_vert[u] lists the non-Dirichlet vertices with slots, cv_ptr[u] ..
+    // cv_ptr[u + 1] their range in cv_slot, cv_count[0] = their number.
+    // Sums over a vertex's slots run in this fixed order (no atomics), so
+    // the contact parts of the gradient, of the diagonal blocks and of every
+    // product are the same bits from run to run.  Rebuilt by the assembly
+    // when the rows changed (cv_version = cons_version of the build).
+    int soff[NTYPES + 1] = {0};
+    DVec<int> cv_keys, cv_vals, cv_keys2, cv_slot, cv_head, cv_vert, cv_ptr, cv_count;
+    DVec<double> cgrad;    // per slot the row's gradient at that vertex (3)
+    unsigned long long cv_version = 0;
     DVec<unsigned long long> keys_pp, keys_pe, keys_tmp;
     DVec<int> counters;    // NTYPES raw append counters + misc
     DVec<unsigned char> cub_tmp, cub_tmp2;  // rocPRIM scan / sort scratch of the first and second stream

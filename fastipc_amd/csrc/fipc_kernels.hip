@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <cstring>
 #include <vector>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include "fipc_ctx.h"
 #include "fipc_rows.h"
 
@@ -183,10 +185,11 @@ __host__ __device__ constexpr int uslot(int a, int c) { return a * 4 - a * (a - 
 // gather reads contiguous runs) and its gradient to
 // tetg[(corner * nt + e) * 3 + i] (corner-major: a wave's stores are
 // contiguous)
+template <class R>
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) k_elastic(int nt, const int* __restrict__ tets, const double* __restrict__ IB,
                                                  const double* __restrict__ vol, const double* __restrict__ la,
                                                  const double* __restrict__ mu, const double* __restrict__ x,
-                                                 double dt2, const int* __restrict__ cpos, double* __restrict__ tetH,
+                                                 double dt2, const int* __restrict__ cpos, R* __restrict__ tetH,
                                                  double* __restrict__ tetg)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -208,15 +211,15 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) k
     // block (a, c), a <= c, belongs to the upper block (min, max) of the global
     // vertices: transposed when the global order is reversed
     tet_hessian_blocks(f, Bm, sc, [&](int a, int c, const double* blk) {
-        double* o = tetH + (size_t)cpos[(size_t)e * 10 + uslot(a, c)] * 9;
+        R* o = tetH + (size_t)cpos[(size_t)e * 10 + uslot(a, c)] * 9;
         if (t[a] <= t[c]) {
 #pragma unroll
-            for (int k = 0; k < 9; k++) o[k] = blk[k];
+            for (int k = 0; k < 9; k++) o[k] = (R)blk[k];
         } else {
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) o[i * 3 + j] = blk[j * 3 + i];
+                for (int j = 0; j < 3; j++) o[i * 3 + j] = (R)blk[j * 3 + i];
         }
     });
 }
@@ -232,8 +235,9 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) k
 constexpr int GATHER_BPW = 7;                       // blocks per wavefront
 constexpr int GATHER_BPG = GATHER_BPW * (TPB / 64); // blocks per workgroup
 static inline int gather_grid(long nnzb) { return (int)((nnzb + GATHER_BPG - 1) / GATHER_BPG); }
+template <class R>
 __global__ void __launch_bounds__(TPB) k_gather_blocks(int nnzb, const int* __restrict__ bcptr,
-                                                       const double* __restrict__ tetH, const int* __restrict__ brow,
+                                                       const R* __restrict__ tetH, const int* __restrict__ brow,
                                                        const int* __restrict__ col, const double* __restrict__ m,
                                                        double* __restrict__ val, float* __restrict__ valf)
 {
@@ -246,9 +250,9 @@ __global__ void __launch_bounds__(TPB) k_gather_blocks(int nnzb, const int* __re
     const int t1 = bcptr[b + 1];
     // unrolled so that several records are in flight (same summation order)
 #pragma unroll 4
-    for (int t = bcptr[b]; t < t1; t++) acc += tetH[(size_t)t * 9 + e];
+    for (int t = bcptr[b]; t < t1; t++) acc += (double)tetH[(size_t)t * 9 + e];
     if ((e == 0 || e == 4 || e == 8) && col[b] == brow[b]) acc += m[brow[b]];
-    val[(size_t)b * 9 + e] = acc;
+    if (val) val[(size_t)b * 9 + e] = acc;
     if (valf) valf[(size_t)b * 9 + e] = (float)acc;
 }
 
@@ -325,9 +329,8 @@ __device__ __forceinline__ void wave_sync()
 template <int KIND>
 __device__ __forceinline__ void contact_row(int n, const int* __restrict__ rows, const double* __restrict__ x,
                                             const double* __restrict__ x0, double dHat2, double kappa,
-                                            double* __restrict__ Hout, double* rhs, double* cdiag, double* Eout,
-                                            double* gout, int project, int accumulate,
-                                            const unsigned char* __restrict__ dmask, int max_sweeps, int gid,
+                                            double* __restrict__ Hout, double* cgrad, double* Eout,
+                                            double* gout, int project, int accumulate, int max_sweeps, int gid,
                                             double* lds, float* __restrict__ Hf, unsigned char* __restrict__ c64)
 {
     using KI = KindInfo<KIND>;
@@ -480,25 +483,15 @@ __device__ __forceinline__ void contact_row(int n, const int* __restrict__ rows,
         for (int k = 0; k < N; k++) Hfr[k] = (float)out[k];
         if (l == 0) c64[r] = constraint_dist(KIND, px) < 1e-4 * dHat2 ? 1 : 0;
     }
-    const int L = l / 3, i = l % 3;
-    int vid = id[0];
     double gl = g[0];
-#pragma unroll
-    for (int a = 1; a < 4; a++)
-        if (a == L) vid = id[a];
 #pragma unroll
     for (int k = 1; k < N; k++)
         if (k == l) gl = g[k];
     if (accumulate) {
-        // Dirichlet rows are replaced by the identity in the solve: skip them
-        // (a fixed plate touches thousands of rows at once)
-        if (dmask[vid]) return;
-        // load_hessian_and_gradient (NEWTON_IPC.py:545-555): rhs -= g, and the
-        // diagonal 3x3 block of the vertex for the preconditioner
-        atomicAdd(&rhs[3 * vid + i], -gl);
-#pragma unroll
-        for (int k = 0; k < N; k++)
-            if (k / 3 == L) atomicAdd(&cdiag[9 * vid + 3 * i + k % 3], out[k]);
+        // the row's gradient at its vertex l / 3 (slot K r + l / 3 of the
+        // type), summed per vertex by k_cv_gather (load_hessian_and_gradient,
+        // NEWTON_IPC.py:546-555)
+        cgrad[3 * (size_t)r * KI::K + l] = gl;
     } else {
         if (l == 0 && Eout) Eout[r] = E;
         if (gout) gout[(size_t)r * N + l] = gl;
@@ -512,13 +505,13 @@ struct ContactBatch {
     double* H[NTYPES];
     float* Hf[NTYPES];            // null unless FP32 copies are wanted
     unsigned char* c64[NTYPES];
+    double* cgrad[NTYPES];        // slot gradients of the type (accumulate mode)
     int n[NTYPES];
     int boff[NTYPES + 1];
 };
 __global__ void __launch_bounds__(TPB) k_contact(ContactBatch cb, const double* __restrict__ x,
-                                                 const double* __restrict__ x0, double dHat2, double kappa, double* rhs,
-                                                 double* cdiag, double* Eout, double* gout, int project,
-                                                 int accumulate, const unsigned char* __restrict__ dmask,
+                                                 const double* __restrict__ x0, double dHat2, double kappa,
+                                                 double* Eout, double* gout, int project, int accumulate,
                                                  int max_sweeps)
 {
     __shared__ double lds[CONTACT_LDS];
@@ -527,8 +520,8 @@ __global__ void __launch_bounds__(TPB) k_contact(ContactBatch cb, const double* 
     int gid = (b - cb.boff[t]) * TPB + (int)threadIdx.x;
 #define CONTACT_CASE(K)                                                                                            \
     case K:                                                                                                         \
-        contact_row<K>(cb.n[K], cb.rows[K], x, x0, dHat2, kappa, cb.H[K], rhs, cdiag, Eout, gout, project,         \
-                       accumulate, dmask, max_sweeps, gid, lds, cb.Hf[K], cb.c64[K]);                               \
+        contact_row<K>(cb.n[K], cb.rows[K], x, x0, dHat2, kappa, cb.H[K], cb.cgrad[K], Eout, gout, project,     \
+                       accumulate, max_sweeps, gid, lds, cb.Hf[K], cb.c64[K]);                                      \
         break;
     switch (t) {
         CONTACT_CASE(0)
@@ -554,9 +547,87 @@ static int contact_blocks(ContactBatch& cb)
     return nb;
 }
 
+// ---- contact vertices: the slots of every row keyed by vertex ----
+// the per-type tables a slot decodes through
+struct ContactSlots {
+    const int* rows[NTYPES];
+    const double* H[NTYPES];
+    const float* Hf[NTYPES];            // FP32 copies (FP32 iterations), or null
+    const unsigned char* c64[NTYPES];   // rows that keep their FP64 Hessian
+    int soff[NTYPES + 1];               // first slot of each type
+};
+__device__ __forceinline__ void slot_decode(const ContactSlots& cs, int s, int& t, int& r, int& a)
+{
+    t = 0;
+    while (t < NTYPES - 1 && s >= cs.soff[t + 1]) t++;
+    const int K = kind_points(t), o = s - cs.soff[t];
+    r = o / K;
+    a = o - r * K;
+}
+// key of a slot: its vertex, or nv for a Dirichlet vertex (sorted last, never summed)
+__global__ void k_cv_keys(int S, ContactSlots cs, const unsigned char* __restrict__ dmask, int nv, int* keys,
+                          int* vals)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    int t, r, a;
+    slot_decode(cs, s, t, r, a);
+    int v = cs.rows[t][kind_points(t) * r + a];
+    keys[s] = dmask[v] ? nv : v;
+    vals[s] = s;
+}
+// head[i] = 1 where a vertex's run starts (head[S] = 0 for the scan's total)
+__global__ void k_cv_heads(int S, int nv, const int* __restrict__ keys, int* head)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > S) return;
+    head[i] = i < S && keys[i] != nv && (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+// pos = exclusive scan of head: run u starts at slot index i with pos[i] = u
+__global__ void k_cv_compact(int S, int nv, const int* __restrict__ keys, const int* __restrict__ pos, int* vert,
+                             int* ptr, int* count)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    const int k = keys[i];
+    if (k == nv) return;
+    const bool head = i == 0 || keys[i - 1] != k;
+    if (head) { vert[pos[i]] = k; ptr[pos[i]] = i; }
+    if (i == S - 1 || keys[i + 1] == nv) {  // the last run ends here
+        const int n = pos[i] + (head ? 1 : 0);
+        ptr[n] = i + 1;
+        count[0] = n;
+    }
+}
+// gradient and diagonal block of every contact vertex, summed over its slots
+// in their fixed order: rhs -= g (after the inertia term), the diagonal block
+// written whole (k_rhs_init cleared every vertex's)
+__global__ void __launch_bounds__(TPB) k_cv_gather(const int* __restrict__ count, const int* __restrict__ vert,
+                                                   const int* __restrict__ ptr, const int* __restrict__ slot,
+                                                   ContactSlots cs, const double* __restrict__ cgrad, double* rhs,
+                                                   double* cdiag)
+{
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    int u = gid / 3, i = gid - 3 * (gid / 3);
+    if (u >= count[0]) return;
+    double g = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int e = ptr[u]; e < ptr[u + 1]; e++) {
+        int sl = slot[e], t, r, a;
+        slot_decode(cs, sl, t, r, a);
+        const int N = kind_dofs(t);
+        g += cgrad[3 * (size_t)sl + i];
+        const double* h = cs.H[t] + (size_t)r * N * N + (size_t)(3 * a + i) * N + 3 * a;
+        d0 += h[0]; d1 += h[1]; d2 += h[2];
+    }
+    const int v = vert[u];
+    rhs[3 * v + i] -= g;
+    cdiag[9 * v + 3 * i] = d0; cdiag[9 * v + 3 * i + 1] = d1; cdiag[9 * v + 3 * i + 2] = d2;
+}
+
 // block-Jacobi preconditioner: inverse of (elastic diag block + contact diag
 // block); Dirichlet vertices get the identity (their residual is zero)
-__global__ void k_precond(int nv, const double* __restrict__ val, const int* __restrict__ diag,
+template <class T>
+__global__ void k_precond(int nv, const T* __restrict__ val, const int* __restrict__ diag,
                           const double* __restrict__ cdiag, const unsigned char* __restrict__ dmask, double* Minv,
                           float* Minvf)
 {
@@ -564,7 +635,7 @@ __global__ void k_precond(int nv, const double* __restrict__ val, const int* __r
     if (i >= nv) return;
     double A[9];
     int b = diag[i];
-    for (int k = 0; k < 9; k++) A[k] = val[(size_t)b * 9 + k] + cdiag[9 * i + k];
+    for (int k = 0; k < 9; k++) A[k] = (double)val[(size_t)b * 9 + k] + cdiag[9 * i + k];
     double* o = Minv + 9 * i;
     double d = det3(A);
     double M[9];
@@ -601,8 +672,9 @@ __global__ void k_precond(int nv, const double* __restrict__ val, const int* __r
 //    a block handles TPB / SPMV_G consecutive rows per pass.  q[row] =
 //    (A_bsr p)[row] is stored (zero on Dirichlet rows), so q needs no
 //    clearing.
-//  - contact rows: a lane group per row, one lane per vertex; the lane's three
-//    rows of H_r p_r are added atomically into qc (skipping Dirichlet vertices).
+//  - contact part: one lane per (contact vertex, component) sums the rows of
+//    H_r p_r at that vertex over its slots in their fixed order and stores
+//    the sum into qc (Dirichlet vertices have no slots).
 // With part != 0 every block stores its partial of p.(A p): p_row . q[row] for
 // BSR rows, p_v . (H_r p_r)_v for contact lanes (Dirichlet entries of the
 // PCG directions are zero, so this is the masked p.q).
@@ -620,58 +692,45 @@ static inline int spmv_grid(int nv)
     int nb = std::min(groups, SPMV_MAXB);
     return std::max(8, (nb + 7) & ~7);
 }
-// contact rows of the seven types, and the first block of each type counted
-// from the first contact block
+// the contact part of a product: the contact vertices with their slot lists
 struct ContactSet {
-    const int* rows[NTYPES];
-    const double* H[NTYPES];
-    const float* Hf[NTYPES];            // FP32 copies (FP32 iterations), or null
-    const unsigned char* c64[NTYPES];   // rows that keep their FP64 Hessian
-    int n[NTYPES];
-    int boff[NTYPES + 1];
+    ContactSlots sl;
+    const int* count;   // number of contact vertices (device)
+    const int* vert;
+    const int* ptr;
+    const int* slot;
 };
-template <int KIND, class V, class H>
-__device__ __forceinline__ double contact_row_dot(const H* __restrict__ Hr, const double (&pv)[3 * KindInfo<KIND>::K],
-                                                  const V* __restrict__ p, double* qc, int v)
+// (sum_r H_r p_r) at contact vertex u, component i: the vertex's slots in
+// their fixed order, each the product of the slot's Hessian row 3 a + i with
+// the row's points.  With Hf set (FP32 iterations) a row uses its FP32 copy
+// unless flagged in c64.
+template <class V>
+__device__ __forceinline__ double contact_vertex_spmv(const ContactSet& cs, const V* __restrict__ p, int u, int i)
 {
-    constexpr int N = KindInfo<KIND>::N;
-    double pq = 0.0;
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        double s = 0.0;
-#pragma unroll
-        for (int c = 0; c < N; c++) s += (double)Hr[i * N + c] * pv[c];
-        atomicAdd(&qc[3 * v + i], s);
-        pq += (double)p[3 * v + i] * s;
+    double s = 0.0;
+    for (int e = cs.ptr[u]; e < cs.ptr[u + 1]; e++) {
+        int t, r, a;
+        slot_decode(cs.sl, cs.slot[e], t, r, a);
+        const int K = kind_points(t), N = 3 * K;
+        const int* rw = cs.sl.rows[t] + K * r;
+        const size_t off = (size_t)r * N * N + (size_t)(3 * a + i) * N;
+        if (cs.sl.Hf[t] && !cs.sl.c64[t][r]) {
+            const float* h = cs.sl.Hf[t] + off;
+            for (int b = 0; b < K; b++) {
+                const int w = rw[b];
+                s += (double)h[3 * b] * (double)p[3 * w] + (double)h[3 * b + 1] * (double)p[3 * w + 1] +
+                     (double)h[3 * b + 2] * (double)p[3 * w + 2];
+            }
+        } else {
+            const double* h = cs.sl.H[t] + off;
+            for (int b = 0; b < K; b++) {
+                const int w = rw[b];
+                s += h[3 * b] * (double)p[3 * w] + h[3 * b + 1] * (double)p[3 * w + 1] +
+                     h[3 * b + 2] * (double)p[3 * w + 2];
+            }
+        }
     }
-    return pq;
-}
-// contact rows of one type: lane `lane` of row r owns the three Hessian rows
-// of the row's vertex `lane`.  With Hf set (FP32 iterations) a row uses its
-// FP32 copy unless flagged in c64.
-template <int KIND, class V>
-__device__ __forceinline__ double contact_rows_spmv(int n, const int* __restrict__ rows, const double* __restrict__ Hc,
-                                                    const float* __restrict__ Hf,
-                                                    const unsigned char* __restrict__ c64,
-                                                    const V* __restrict__ p, double* qc,
-                                                    const unsigned char* __restrict__ dmask, int gid)
-{
-    using KI = KindInfo<KIND>;
-    constexpr int N = KI::N, K = KI::K;
-    constexpr int W = KIND == 0 ? 2 : (KIND == 1 ? 3 : 4);
-    int r = gid / K, lane = gid % K;
-    if (r >= n) return 0.0;
-    double pv[N];
-#pragma unroll
-    for (int a = 0; a < K; a++) {
-        int v = rows[W * r + a];
-        pv[3 * a] = p[3 * v]; pv[3 * a + 1] = p[3 * v + 1]; pv[3 * a + 2] = p[3 * v + 2];
-    }
-    int v = rows[W * r + lane];
-    if (dmask[v]) return 0.0;
-    const size_t off = (size_t)r * N * N + lane * 3 * N;
-    if (Hf && !c64[r]) return contact_row_dot<KIND, V, float>(Hf + off, pv, p, qc, v);
-    return contact_row_dot<KIND, V, double>(Hc + off, pv, p, qc, v);
+    return s;
 }
 template <class T, class V>
 __global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ rowptr, const int* __restrict__ col,
@@ -754,18 +813,15 @@ __global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ ro
             }
         }
     } else {
-        int cb = (int)blockIdx.x;
-        int t = 0;
-        while (t < NTYPES - 1 && cb >= cs.boff[t + 1]) t++;
-        int gid = (cb - cs.boff[t]) * TPB + (int)threadIdx.x;
-        switch (t) {
-        case 0: pq = contact_rows_spmv<0, V>(cs.n[0], cs.rows[0], cs.H[0], cs.Hf[0], cs.c64[0], p, qc, dmask, gid); break;
-        case 1: pq = contact_rows_spmv<1, V>(cs.n[1], cs.rows[1], cs.H[1], cs.Hf[1], cs.c64[1], p, qc, dmask, gid); break;
-        case 2: pq = contact_rows_spmv<2, V>(cs.n[2], cs.rows[2], cs.H[2], cs.Hf[2], cs.c64[2], p, qc, dmask, gid); break;
-        case 3: pq = contact_rows_spmv<3, V>(cs.n[3], cs.rows[3], cs.H[3], cs.Hf[3], cs.c64[3], p, qc, dmask, gid); break;
-        case 4: pq = contact_rows_spmv<4, V>(cs.n[4], cs.rows[4], cs.H[4], cs.Hf[4], cs.c64[4], p, qc, dmask, gid); break;
-        case 5: pq = contact_rows_spmv<5, V>(cs.n[5], cs.rows[5], cs.H[5], cs.Hf[5], cs.c64[5], p, qc, dmask, gid); break;
-        default: pq = contact_rows_spmv<6, V>(cs.n[6], cs.rows[6], cs.H[6], cs.Hf[6], cs.c64[6], p, qc, dmask, gid); break;
+        // contact part: one lane per (contact vertex, component); the sum is
+        // stored (each entry of qc has one writer), not added
+        int gid = (int)blockIdx.x * TPB + (int)threadIdx.x;
+        int u = gid / 3, i = gid - 3 * (gid / 3);
+        if (u < cs.count[0]) {
+            double sv = contact_vertex_spmv<V>(cs, p, u, i);
+            int v = cs.vert[u];
+            qc[3 * v + i] = sv;
+            pq = (double)p[3 * v + i] * sv;
         }
     }
     if (part) {
@@ -1171,10 +1227,12 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->x); release(c->x0); release(c->xn); release(c->xt); release(c->xprev); release(c->v);
     release(c->sol); release(c->rhs); release(c->target);
     release(c->rowptr); release(c->col); release(c->diag); release(c->val); release(c->valf);
-    release(c->brow); release(c->tetH); release(c->tetg); release(c->bcptr); release(c->cpos); release(c->vcptr);
+    release(c->brow); release(c->tetH); release(c->tetHf); release(c->tetg); release(c->bcptr); release(c->cpos); release(c->vcptr);
     release(c->lptr); release(c->lent);
     release(c->vcent);
     for (int t = 0; t < NTYPES; t++) { release(c->rows[t]); release(c->cH[t]); release(c->cHf[t]); release(c->c64[t]); }
+    release(c->cv_keys); release(c->cv_vals); release(c->cv_keys2); release(c->cv_slot); release(c->cv_head);
+    release(c->cv_vert); release(c->cv_ptr); release(c->cv_count); release(c->cgrad);
     release(c->keys_pp); release(c->keys_pe); release(c->keys_tmp); release(c->counters); release(c->cub_tmp); release(c->cub_tmp2); release(c->rows_tmp);
     release(c->nuniq); release(c->cdiag); release(c->Minv); release(c->r); release(c->z); release(c->pp); release(c->q); release(c->qc);
     release(c->Minvf); release(c->rf); release(c->zf); release(c->pf); release(c->qf);
@@ -1337,6 +1395,7 @@ extern "C" int fipc_upload_mesh(fipc_ctx* c, int nv, const double* x0, const dou
         STCHK(ensure(c->cH[t], c->cap[t] * N * N));
         c->count[t] = 0;
     }
+    c->cons_version++;  // no rows (and a new Dirichlet mask) from here
     return build_pattern(c, nv, nt, tets);
 }
 
@@ -1480,8 +1539,65 @@ extern "C" int fipc_residual(fipc_ctx* c, double* res)
 }
 
 // ---- assembly ----
+static ContactSlots contact_slots(fipc_ctx* c, bool fp32)
+{
+    ContactSlots sl;
+    for (int t = 0; t < NTYPES; t++) {
+        sl.rows[t] = c->rows[t].p;
+        sl.H[t] = c->cH[t].p;
+        sl.Hf[t] = fp32 ? c->cHf[t].p : nullptr;
+        sl.c64[t] = fp32 ? c->c64[t].p : nullptr;
+        sl.soff[t] = c->soff[t];
+    }
+    sl.soff[NTYPES] = c->soff[NTYPES];
+    return sl;
+}
+// the contact vertices of the current rows (see fipc_ctx): slot keys, a
+// stable radix sort by vertex, run heads, their scan and the compaction; no
+// host synchronisation (the vertex count stays on the device)
+static int build_contact_vertices(fipc_ctx* c, hipStream_t st)
+{
+    int S = 0;
+    for (int t = 0; t < NTYPES; t++) { c->soff[t] = S; S += c->count[t] * kind_points(t); }
+    c->soff[NTYPES] = S;
+    STCHK(ensure(c->cv_count, 1));
+    STCHK(ensure(c->cgrad, (size_t)std::max(S, 1) * 3));
+    c->cv_version = c->cons_version;
+    if (S == 0) {
+        HIPCHK(hipMemsetAsync(c->cv_count.p, 0, 4, st));
+        return FIPC_OK;
+    }
+    STCHK(ensure(c->cv_keys, S)); STCHK(ensure(c->cv_vals, S)); STCHK(ensure(c->cv_keys2, S));
+    STCHK(ensure(c->cv_slot, S)); STCHK(ensure(c->cv_head, (size_t)S + 1));
+    STCHK(ensure(c->cv_vert, S)); STCHK(ensure(c->cv_ptr, (size_t)S + 1));
+    HIPCHK(hipMemsetAsync(c->cv_count.p, 0, 4, st));
+    k_cv_keys<<<nblk(S), TPB, 0, st>>>(S, contact_slots(c, false), c->dmask.p, c->nv, c->cv_keys.p, c->cv_vals.p);
+    HIPCHK(hipGetLastError());
+    unsigned bits = 1;
+    while ((1u << bits) <= (unsigned)c->nv) bits++;  // keys up to nv
+    size_t b1 = 0, b2 = 0;
+    HIPCHK(rocprim::radix_sort_pairs(nullptr, b1, c->cv_keys.p, c->cv_keys2.p, c->cv_vals.p, c->cv_slot.p, S, 0, bits,
+                                     st));
+    HIPCHK(rocprim::exclusive_scan(nullptr, b2, c->cv_head.p, c->cv_head.p, 0, (size_t)S + 1, rocprim::plus<int>(),
+                                   st));
+    size_t need = std::max(b1, b2);
+    if (need > c->cub_tmp2.cap) STCHK(ensure(c->cub_tmp2, need));
+    b1 = c->cub_tmp2.cap;
+    HIPCHK(rocprim::radix_sort_pairs(c->cub_tmp2.p, b1, c->cv_keys.p, c->cv_keys2.p, c->cv_vals.p, c->cv_slot.p, S, 0,
+                                     bits, st));
+    k_cv_heads<<<nblk((long)S + 1), TPB, 0, st>>>(S, c->nv, c->cv_keys2.p, c->cv_head.p);
+    HIPCHK(hipGetLastError());
+    b2 = c->cub_tmp2.cap;
+    HIPCHK(rocprim::exclusive_scan(c->cub_tmp2.p, b2, c->cv_head.p, c->cv_head.p, 0, (size_t)S + 1,
+                                   rocprim::plus<int>(), st));
+    k_cv_compact<<<nblk(S), TPB, 0, st>>>(S, c->nv, c->cv_keys2.p, c->cv_head.p, c->cv_vert.p, c->cv_ptr.p,
+                                         c->cv_count.p);
+    HIPCHK(hipGetLastError());
+    return FIPC_OK;
+}
 static int launch_contacts(fipc_ctx* c, int accumulate, hipStream_t st)
 {
+    if (c->cv_version != c->cons_version) STCHK(build_contact_vertices(c, st));
     ContactBatch cb;
     for (int t = 0; t < NTYPES; t++) {
         cb.rows[t] = c->rows[t].p;
@@ -1494,11 +1610,15 @@ static int launch_contacts(fipc_ctx* c, int accumulate, hipStream_t st)
         }
         cb.Hf[t] = c->precision == 1 ? c->cHf[t].p : nullptr;
         cb.c64[t] = c->precision == 1 ? c->c64[t].p : nullptr;
+        cb.cgrad[t] = c->cgrad.p + 3 * (size_t)c->soff[t];
     }
     int nb = contact_blocks(cb);
     if (!nb) return FIPC_OK;
-    k_contact<<<nb, TPB, 0, st>>>(cb, c->x.p, c->x0.p, c->dHat2, c->kappa, c->rhs.p, c->cdiag.p, nullptr, nullptr, 1,
-                                  accumulate, c->dmask.p, 20);
+    k_contact<<<nb, TPB, 0, st>>>(cb, c->x.p, c->x0.p, c->dHat2, c->kappa, nullptr, nullptr, 1, accumulate, 20);
+    HIPCHK(hipGetLastError());
+    const int S = c->soff[NTYPES];
+    k_cv_gather<<<nblk(3L * S), TPB, 0, st>>>(c->cv_count.p, c->cv_vert.p, c->cv_ptr.p, c->cv_slot.p,
+                                              contact_slots(c, false), c->cgrad.p, c->rhs.p, c->cdiag.p);
     HIPCHK(hipGetLastError());
     return FIPC_OK;
 }
@@ -1508,8 +1628,14 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     if (!c || !c->nv) return FIPC_ERR_STATE;
     c->precision = precision;
     c->sys_version++;
-    if (precision == 1) STCHK(ensure(c->valf, (size_t)c->nnzb * 9));
-    if (precision == 1) STCHK(ensure(c->Minvf, (size_t)c->nv * 9));
+    if (precision == 1) {
+        STCHK(ensure(c->valf, (size_t)c->nnzb * 9));
+        STCHK(ensure(c->Minvf, (size_t)c->nv * 9));
+        STCHK(ensure(c->tetHf, (size_t)std::max(c->nt, 1) * 90));
+    } else {
+        STCHK(ensure(c->val, (size_t)c->nnzb * 9));
+        STCHK(ensure(c->tetH, (size_t)std::max(c->nt, 1) * 90));
+    }
     // the contact rows (few, latency-bound) run on the second stream while
     // the tet pass and the block gather fill the GPU: the second stream
     // starts after the inertia term and the cleared diagonal blocks, and the
@@ -1522,42 +1648,55 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     STCHK(launch_contacts(c, 1, c->s2));
     STCHK(fipc_tm_end(c, TM_CONTACT, c->s2));
     HIPCHK(hipEventRecord(c->ev_contact, c->s2));
+    // FP64: the per-tet records and the block values are FP64.  FP32: the
+    // records are rounded to FP32 and each block sums them in FP64 and
+    // stores the FP32 value only (the refinement's residual products read
+    // the same FP32 values in FP64 arithmetic)
     STCHK(fipc_tm_begin(c, TM_ELASTIC));
-    if (c->nt)
-        k_elastic<<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p, c->x.p,
-                                                  c->dt * c->dt, c->cpos.p, c->tetH.p, c->tetg.p);
+    if (c->nt) {
+        if (precision == 1)
+            k_elastic<float><<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p,
+                                                             c->x.p, c->dt * c->dt, c->cpos.p, c->tetHf.p, c->tetg.p);
+        else
+            k_elastic<double><<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p,
+                                                              c->x.p, c->dt * c->dt, c->cpos.p, c->tetH.p, c->tetg.p);
+    }
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_ELASTIC));
     STCHK(fipc_tm_begin(c, TM_GATHER));
-    k_gather_blocks<<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->tetH.p, c->brow.p, c->col.p, c->m.p,
-                                                     c->val.p, precision == 1 ? c->valf.p : nullptr);
+    if (precision == 1)
+        k_gather_blocks<float><<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->tetHf.p, c->brow.p,
+                                                                        c->col.p, c->m.p, nullptr, c->valf.p);
+    else
+        k_gather_blocks<double><<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->tetH.p, c->brow.p,
+                                                                         c->col.p, c->m.p, c->val.p, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamWaitEvent(c->s, c->ev_contact, 0));
     k_gather_tetg<<<rhs_grid(c->nv), TPB, 0, c->s>>>(c->nv, c->vcptr.p, c->vcent.p, c->tetg.p, c->dmask.p, c->rhs.p);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_GATHER));
-    k_precond<<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->val.p, c->diag.p, c->cdiag.p, c->dmask.p, c->Minv.p,
-                                             precision == 1 ? c->Minvf.p : nullptr);
+    if (precision == 1)
+        k_precond<float><<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->valf.p, c->diag.p, c->cdiag.p, c->dmask.p,
+                                                         c->Minv.p, c->Minvf.p);
+    else
+        k_precond<double><<<nblk(c->nv), TPB, 0, c->s>>>(c->nv, c->val.p, c->diag.p, c->cdiag.p, c->dmask.p,
+                                                          c->Minv.p, nullptr);
     HIPCHK(hipGetLastError());
     return FIPC_OK;
 }
 
 // ---- SpMV ----
 // contact blocks follow the BSR blocks; returns the total grid size
+// the contact vertices of the last assembly (an upper bound of their number,
+// the slot count, sizes the grid; the lanes past the device count exit)
 static int spmv_contacts(fipc_ctx* c, ContactSet& cs, bool fp32 = false)
 {
-    int nb = 0;
-    for (int t = 0; t < NTYPES; t++) {
-        int K = t == 0 ? 2 : (t == 1 ? 3 : 4);
-        cs.rows[t] = c->rows[t].p;
-        cs.H[t] = c->cH[t].p;
-        cs.Hf[t] = fp32 ? c->cHf[t].p : nullptr;
-        cs.c64[t] = fp32 ? c->c64[t].p : nullptr;
-        cs.n[t] = c->count[t];
-        cs.boff[t] = nb;
-        nb += nblk((long)c->count[t] * K);
-    }
-    cs.boff[NTYPES] = nb;
+    cs.sl = contact_slots(c, fp32);
+    cs.count = c->cv_count.p;
+    cs.vert = c->cv_vert.p;
+    cs.ptr = c->cv_ptr.p;
+    cs.slot = c->cv_slot.p;
+    int nb = c->soff[NTYPES] ? nblk(3L * c->soff[NTYPES]) : 0;
     return (nb + 7) & ~7;
 }
 // q + qc = A p (q = BSR part, qc = contact part accumulated: qc must be zero
@@ -1565,6 +1704,9 @@ static int spmv_contacts(fipc_ctx* c, ContactSet& cs, bool fp32 = false)
 template <class V>
 static int spmv(fipc_ctx* c, int prec, const V* p, V* q, double* qc, double* part, const int* state, int it)
 {
+    // the contact vertices (and Hessians) are those of the last assembly:
+    // rows found since then have none yet
+    if (c->cv_version != c->cons_version) return FIPC_ERR_STATE;
     ContactSet cs;
     // the FP32 iterations (FP32 vectors) read the FP32 contact copies
     int nbc = spmv_contacts(c, cs, prec == 1 && sizeof(V) == 4);
@@ -1600,7 +1742,7 @@ extern "C" int fipc_spmv(fipc_ctx* c, const double* x, double* y)
     size_t n = (size_t)c->nv * 24;
     HIPCHK(hipMemcpyAsync(c->pp.p, x, n, hipMemcpyHostToDevice, c->s));
     HIPCHK(hipMemsetAsync(c->qc.p, 0, n, c->s));
-    STCHK(spmv(c, 0, c->pp.p, c->q.p, c->qc.p, nullptr, nullptr, 0));
+    STCHK(spmv(c, c->precision, c->pp.p, c->q.p, c->qc.p, nullptr, nullptr, 0));
     k_add<<<nblk((long)c->nv * 3), TPB, 0, c->s>>>(c->nv * 3, c->qc.p, c->q.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(y, c->q.p, n, hipMemcpyDeviceToHost, c->s));
@@ -1628,7 +1770,7 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
         // r = b - A x with the FP64 blocks (q + qc), z = Minv r, p = z; in
         // round 0 the product exits at once when x0 = 0 (the flag is -1)
         const int* x0zero = round == 0 ? c->x0flag.p : nullptr;
-        STCHK(spmv<double>(c, 0, c->sol.p, c->q.p, c->qc.p, nullptr, x0zero, 0));
+        STCHK(spmv<double>(c, c->precision, c->sol.p, c->q.p, c->qc.p, nullptr, x0zero, 0));
         k_pcg_init<V><<<nbv, TPB, 0, c->s>>>(nv, c->rhs.p, c->q.p, c->qc.p, c->dmask.p, c->Minv.p, r, z, p,
                                              c->part_a.p, c->part_b.p, x0zero);
         HIPCHK(hipGetLastError());
@@ -2032,10 +2174,10 @@ extern "C" int fipc_batch_constraint(int type, int n, const double* x, const dou
     ContactBatch cb;
     for (int t = 0; t < NTYPES; t++) {
         cb.rows[t] = drows; cb.H[t] = dH; cb.n[t] = t == type ? n : 0;
-        cb.Hf[t] = nullptr; cb.c64[t] = nullptr;
+        cb.Hf[t] = nullptr; cb.c64[t] = nullptr; cb.cgrad[t] = nullptr;
     }
     int nb = contact_blocks(cb);
-    k_contact<<<nb, TPB>>>(cb, dx, dx0, dHat2, kappa, nullptr, nullptr, dE, dg, project, 0, nullptr, 20);
+    k_contact<<<nb, TPB>>>(cb, dx, dx0, dHat2, kappa, dE, dg, project, 0, 20);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     if (E) HIPCHK(hipMemcpy(E, dE, (size_t)n * 8, hipMemcpyDeviceToHost));

@@ -227,6 +227,8 @@ def ref():
             f.argtypes = [_D] * (n * n + n) + [_P] * n
         r.svd_2.restype = None
         r.svd_2.argtypes = [_D] * 4 + [_P] * 12
+        r.svd_3.restype = None
+        r.svd_3.argtypes = [_D] * 9 + [_P] * 27
         r.point_triangle_ccd.restype = None
         r.point_triangle_ccd.argtypes = [_D] * 26 + [_P]
         r.edge_edge_ccd.restype = None
@@ -259,6 +261,13 @@ def ref_svd2(F):
     ref().svd_2(*[float(v) for v in np.ravel(F)], *[C.byref(o) for o in outs])
     v = np.array([o.value for o in outs])
     return v[:4].reshape(2, 2), v[4:8].reshape(2, 2), v[8:].reshape(2, 2)
+
+
+def ref_svd3(F):
+    outs = [C.c_double() for _ in range(27)]
+    ref().svd_3(*[float(v) for v in np.ravel(F)], *[C.byref(o) for o in outs])
+    v = np.array([o.value for o in outs])
+    return v[:9].reshape(3, 3), v[9:18].reshape(3, 3), v[18:].reshape(3, 3)
 
 
 def ref_pt_ccd(p, t0, t1, t2, dp, dt0, dt1, dt2, eta, dist2):

@@ -8,8 +8,9 @@
   and of the neo-Hookean energy Psi(F) (common/physics/neo_hookean.py:5-16):
   inputs, values, gradients, Hessians.  The formulas here are the plain
   definitions; derivatives come from torch, independent of the oracle's jets.
-* reference_dense.npz -- solve_N, svd_2 and project_pd_{2,4} of the same
-  native library, on seeded SPD / indefinite / diagonal inputs;
+* reference_dense.npz (or alone with --dense) -- solve_N, svd_2, svd_3 and
+  project_pd_{2,4} of the same native library, on seeded SPD / indefinite /
+  diagonal / rank-deficient inputs;
 * reference_ccd_scenes.npz (--scene-ccd) -- the reference's CCD step bounds
   on the candidate pairs of stacked-cube and slab states (scene_ccd_set);
 * reference.npz -- outputs of the reference's own native library
@@ -204,7 +205,7 @@ def reference_set(rng):
 
 
 def dense_set(rng):
-    """solve_N, svd_2 and project_pd_{2,4} of the reference's native library."""
+    """solve_N, svd_2, project_pd_{2,4} and svd_3 of the reference's native library."""
     from oracle import lib as O
     if O.ref() is None:
         return None
@@ -232,6 +233,20 @@ def dense_set(rng):
         A = A + np.transpose(A, (0, 2, 1))
         out[f"pd{n}_in"] = A
         out[f"pd{n}_out"] = np.array([O.ref_project_pd(a) for a in A])
+    # svd_3: general, diagonal (signs), rank-deficient and repeated-value inputs
+    F = list(rng.normal(size=(200, 3, 3)))
+    F += [np.diag(rng.normal(size=3)) for _ in range(20)]
+    F += [rng.normal(size=(3, 2)) @ rng.normal(size=(2, 3)) for _ in range(20)]
+    for _ in range(20):
+        Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        R, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        F.append(Q @ np.diag([2.0, 2.0, 1.0]) @ R)
+    F = np.array(F)
+    out["svd3_F"] = F
+    r = [O.ref_svd3(f) for f in F]
+    out["svd3_U"] = np.array([u for u, _, _ in r])
+    out["svd3_S"] = np.array([s for _, s, _ in r])
+    out["svd3_V"] = np.array([v for _, _, v in r])
     return out
 
 
@@ -328,6 +343,11 @@ def scene_ccd_set(rng):
 if __name__ == "__main__" and "--scene-ccd" in sys.argv:
     np.savez_compressed(os.path.join(HERE, "reference_ccd_scenes.npz"), **scene_ccd_set(np.random.default_rng(31)))
     print("wrote reference_ccd_scenes.npz")
+    sys.exit(0)
+
+if __name__ == "__main__" and "--dense" in sys.argv:
+    np.savez_compressed(os.path.join(HERE, "reference_dense.npz"), **dense_set(np.random.default_rng(777)))
+    print("wrote reference_dense.npz")
     sys.exit(0)
 
 if __name__ == "__main__":

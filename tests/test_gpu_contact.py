@@ -193,3 +193,26 @@ def _body_sizes(sc):
     change = np.flatnonzero(np.diff(lab) != 0) + 1
     edges = np.r_[0, change, n]
     return np.diff(edges)
+
+
+@pytest.mark.parametrize("cfg,steps,precision", [(1, 50, 0), (2, 60, 1)])
+def test_contact_steps_are_bit_reproducible(cfg, steps, precision):
+    """Two runs of the same scene into contact give the same bits: the
+    contact gradient, diagonal blocks and products sum each vertex's rows
+    in a fixed order (no atomics), so the Newton iterates, PCG iteration
+    counts and step sizes repeat exactly."""
+    sc = S.config(cfg)
+    runs = []
+    for _ in range(2):
+        g = NewtonIPC(sc, precision=precision)
+        hist = []
+        for f in range(steps):
+            st = g.step()
+            hist.append((st.newton_iters, tuple(st.pcg_iters), tuple(st.alphas)))
+        runs.append((np.array(g.x), hist, {k: len(v) for k, v in g.constraints().items()}))
+        del g
+    (xa, ha, ca), (xb, hb, cb) = runs
+    print(f"config {cfg} precision {precision}: contacts {ca}")
+    assert sum(ca.values()) > 0
+    assert ha == hb
+    assert np.array_equal(xa.view(np.uint64), xb.view(np.uint64))
