@@ -599,29 +599,51 @@ __global__ void k_cv_compact(int S, int nv, const int* __restrict__ keys, const 
         count[0] = n;
     }
 }
+// lanes per contact vertex: lane l of a vertex's group takes its slots
+// l, l + CV_G, ... in order, and the group combines the lanes' sums with a
+// fixed butterfly, so a vertex with many slots costs a few dependent loads,
+// not one chain per slot, and the result is the same bits on every run
+#define CV_G 8
 // gradient and diagonal block of every contact vertex, summed over its slots
-// in their fixed order: rhs -= g (after the inertia term), the diagonal block
-// written whole (k_rhs_init cleared every vertex's)
+// (CV_G lanes per vertex): rhs -= g (after the inertia term), the diagonal
+// block written whole (k_rhs_init cleared every vertex's)
 __global__ void __launch_bounds__(TPB) k_cv_gather(const int* __restrict__ count, const int* __restrict__ vert,
                                                    const int* __restrict__ ptr, const int* __restrict__ slot,
                                                    ContactSlots cs, const double* __restrict__ cgrad, double* rhs,
                                                    double* cdiag)
 {
-    int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    int u = gid / 3, i = gid - 3 * (gid / 3);
-    if (u >= count[0]) return;
-    double g = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
-    for (int e = ptr[u]; e < ptr[u + 1]; e++) {
-        int sl = slot[e], t, r, a;
-        slot_decode(cs, sl, t, r, a);
-        const int N = kind_dofs(t);
-        g += cgrad[3 * (size_t)sl + i];
-        const double* h = cs.H[t] + (size_t)r * N * N + (size_t)(3 * a + i) * N + 3 * a;
-        d0 += h[0]; d1 += h[1]; d2 += h[2];
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int u = gid / CV_G, l = gid % CV_G;
+    const bool live = u < count[0];
+    double acc[12];  // g (3), then the diagonal block row-major (9)
+#pragma unroll
+    for (int k = 0; k < 12; k++) acc[k] = 0.0;
+    if (live) {
+        for (int e = ptr[u] + l; e < ptr[u + 1]; e += CV_G) {
+            int sl = slot[e], t, r, a;
+            slot_decode(cs, sl, t, r, a);
+            const int N = kind_dofs(t);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                acc[i] += cgrad[3 * (size_t)sl + i];
+                const double* h = cs.H[t] + (size_t)r * N * N + (size_t)(3 * a + i) * N + 3 * a;
+                acc[3 + 3 * i] += h[0]; acc[4 + 3 * i] += h[1]; acc[5 + 3 * i] += h[2];
+            }
+        }
     }
-    const int v = vert[u];
+    // every lane of the wavefront takes part (groups never straddle one)
+#pragma unroll
+    for (int o = CV_G / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int k = 0; k < 12; k++) acc[k] += __shfl_xor(acc[k], o, 64);
+    if (!live || l >= 3) return;
+    const int v = vert[u], i = l;
+    double g = i == 0 ? acc[0] : (i == 1 ? acc[1] : acc[2]);
+    double d[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) d[k] = i == 0 ? acc[3 + k] : (i == 1 ? acc[6 + k] : acc[9 + k]);
     rhs[3 * v + i] -= g;
-    cdiag[9 * v + 3 * i] = d0; cdiag[9 * v + 3 * i + 1] = d1; cdiag[9 * v + 3 * i + 2] = d2;
+    cdiag[9 * v + 3 * i] = d[0]; cdiag[9 * v + 3 * i + 1] = d[1]; cdiag[9 * v + 3 * i + 2] = d[2];
 }
 
 // block-Jacobi preconditioner: inverse of (elastic diag block + contact diag
@@ -700,37 +722,46 @@ struct ContactSet {
     const int* ptr;
     const int* slot;
 };
-// (sum_r H_r p_r) at contact vertex u, component i: the vertex's slots in
-// their fixed order, each the product of the slot's Hessian row 3 a + i with
-// the row's points.  With Hf set (FP32 iterations) a row uses its FP32 copy
-// unless flagged in c64.
+// lane l's share of (sum_r H_r p_r) at contact vertex u: the vertex's slots
+// l, l + CV_G, ..., each the product of the slot's three Hessian rows 3 a + i
+// with the row's points, added into s[3].  With Hf set (FP32 iterations) a
+// row uses its FP32 copy unless flagged in c64.
 template <class V>
-__device__ __forceinline__ double contact_vertex_spmv(const ContactSet& cs, const V* __restrict__ p, int u, int i)
+__device__ __forceinline__ void contact_vertex_spmv(const ContactSet& cs, const V* __restrict__ p, int u, int l,
+                                                    double* s)
 {
-    double s = 0.0;
-    for (int e = cs.ptr[u]; e < cs.ptr[u + 1]; e++) {
+    for (int e = cs.ptr[u] + l; e < cs.ptr[u + 1]; e += CV_G) {
         int t, r, a;
         slot_decode(cs.sl, cs.slot[e], t, r, a);
         const int K = kind_points(t), N = 3 * K;
         const int* rw = cs.sl.rows[t] + K * r;
-        const size_t off = (size_t)r * N * N + (size_t)(3 * a + i) * N;
-        if (cs.sl.Hf[t] && !cs.sl.c64[t][r]) {
-            const float* h = cs.sl.Hf[t] + off;
-            for (int b = 0; b < K; b++) {
+        const size_t off = (size_t)r * N * N + (size_t)(3 * a) * N;
+        // point by point (static indices, guarded up to the widest row), so
+        // only one point's p and the three partial rows are live
+        const bool f32 = cs.sl.Hf[t] && !cs.sl.c64[t][r];
+        const float* hf = f32 ? cs.sl.Hf[t] + off : nullptr;
+        const double* hd = cs.sl.H[t] + off;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            if (b < K) {
                 const int w = rw[b];
-                s += (double)h[3 * b] * (double)p[3 * w] + (double)h[3 * b + 1] * (double)p[3 * w + 1] +
-                     (double)h[3 * b + 2] * (double)p[3 * w + 2];
-            }
-        } else {
-            const double* h = cs.sl.H[t] + off;
-            for (int b = 0; b < K; b++) {
-                const int w = rw[b];
-                s += h[3 * b] * (double)p[3 * w] + h[3 * b + 1] * (double)p[3 * w + 1] +
-                     h[3 * b + 2] * (double)p[3 * w + 2];
+                const double p0 = (double)p[3 * w], p1 = (double)p[3 * w + 1], p2 = (double)p[3 * w + 2];
+                if (f32) {
+                    const float* h = hf + 3 * b;
+                    a0 += (double)h[0] * p0 + (double)h[1] * p1 + (double)h[2] * p2;
+                    a1 += (double)h[N] * p0 + (double)h[N + 1] * p1 + (double)h[N + 2] * p2;
+                    a2 += (double)h[2 * N] * p0 + (double)h[2 * N + 1] * p1 + (double)h[2 * N + 2] * p2;
+                } else {
+                    const double* h = hd + 3 * b;
+                    a0 += h[0] * p0 + h[1] * p1 + h[2] * p2;
+                    a1 += h[N] * p0 + h[N + 1] * p1 + h[N + 2] * p2;
+                    a2 += h[2 * N] * p0 + h[2 * N + 1] * p1 + h[2 * N + 2] * p2;
+                }
             }
         }
+        s[0] += a0; s[1] += a1; s[2] += a2;
     }
-    return s;
 }
 template <class T, class V>
 __global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ rowptr, const int* __restrict__ col,
@@ -813,15 +844,24 @@ __global__ void __launch_bounds__(TPB) k_spmv(int nv, const int* __restrict__ ro
             }
         }
     } else {
-        // contact part: one lane per (contact vertex, component); the sum is
-        // stored (each entry of qc has one writer), not added
-        int gid = (int)blockIdx.x * TPB + (int)threadIdx.x;
-        int u = gid / 3, i = gid - 3 * (gid / 3);
-        if (u < cs.count[0]) {
-            double sv = contact_vertex_spmv<V>(cs, p, u, i);
-            int v = cs.vert[u];
-            qc[3 * v + i] = sv;
-            pq = (double)p[3 * v + i] * sv;
+        // contact part: CV_G lanes per contact vertex, combined by a fixed
+        // butterfly; the sum is stored (each entry of qc has one writer)
+        const int gid = (int)blockIdx.x * TPB + (int)threadIdx.x;
+        const int u = gid / CV_G, l = gid % CV_G;
+        const bool live = u < cs.count[0];
+        double sv[3] = {0.0, 0.0, 0.0};
+        if (live) contact_vertex_spmv<V>(cs, p, u, l, sv);
+#pragma unroll
+        for (int o = CV_G / 2; o > 0; o >>= 1) {
+            sv[0] += __shfl_xor(sv[0], o, 64);
+            sv[1] += __shfl_xor(sv[1], o, 64);
+            sv[2] += __shfl_xor(sv[2], o, 64);
+        }
+        if (live && l < 3) {
+            const int v = cs.vert[u];
+            const double si = l == 0 ? sv[0] : (l == 1 ? sv[1] : sv[2]);
+            qc[3 * v + l] = si;
+            pq = (double)p[3 * v + l] * si;
         }
     }
     if (part) {
@@ -1617,7 +1657,7 @@ static int launch_contacts(fipc_ctx* c, int accumulate, hipStream_t st)
     k_contact<<<nb, TPB, 0, st>>>(cb, c->x.p, c->x0.p, c->dHat2, c->kappa, nullptr, nullptr, 1, accumulate, 20);
     HIPCHK(hipGetLastError());
     const int S = c->soff[NTYPES];
-    k_cv_gather<<<nblk(3L * S), TPB, 0, st>>>(c->cv_count.p, c->cv_vert.p, c->cv_ptr.p, c->cv_slot.p,
+    k_cv_gather<<<nblk((long)CV_G * S), TPB, 0, st>>>(c->cv_count.p, c->cv_vert.p, c->cv_ptr.p, c->cv_slot.p,
                                               contact_slots(c, false), c->cgrad.p, c->rhs.p, c->cdiag.p);
     HIPCHK(hipGetLastError());
     return FIPC_OK;
@@ -1696,7 +1736,7 @@ static int spmv_contacts(fipc_ctx* c, ContactSet& cs, bool fp32 = false)
     cs.vert = c->cv_vert.p;
     cs.ptr = c->cv_ptr.p;
     cs.slot = c->cv_slot.p;
-    int nb = c->soff[NTYPES] ? nblk(3L * c->soff[NTYPES]) : 0;
+    int nb = c->soff[NTYPES] ? nblk((long)CV_G * c->soff[NTYPES]) : 0;
     return (nb + 7) & ~7;
 }
 // q + qc = A p (q = BSR part, qc = contact part accumulated: qc must be zero
