@@ -229,9 +229,15 @@ static int grid_build(fipc_ctx* c, int slot, int n, const BoxFn& box, hipStream_
 {
     const hipStream_t s = st ? st : c->s;
     Grid& g = c->grid[slot];
-    // the same boxes at the same x: the grid is already there
-    if (g.reusable && !box.sol && g.x_version == c->x_version && g.kind == box.kind && g.n == n && g.infl == box.infl)
-        return FIPC_OK;
+    // the same boxes at the same x: the grid is already there; or the grid
+    // sweeps these boxes from the x of version g.x_version along the current
+    // sol, and x lies on that segment
+    if (g.reusable && !box.sol && g.kind == box.kind && g.n == n && g.infl == box.infl) {
+        if (!g.swept && g.x_version == c->x_version) return FIPC_OK;
+        if (g.swept && c->seg_x == c->x_version && c->seg_base == g.x_version && c->seg_sol == g.seg_sol &&
+            c->seg_sol == c->sol_version)
+            return FIPC_OK;
+    }
     g.reusable = false;
     STCHK(fipc_tm_begin(c, TM_GRID, s));
     int cap = g.cap;
@@ -256,8 +262,11 @@ static int grid_build(fipc_ctx* c, int slot, int n, const BoxFn& box, hipStream_
     k_grid_scatter<<<nb, TPB, 0, s>>>(nep, cap, g.ekey.p, g.eitem.p, g.ecell.p, g.bstart.p, g.items.p, g.cells.p);
     HIPCHK(hipGetLastError());
     // reusable once built; an overflow found after the operation's
-    // synchronisation clears this again (bp_check)
-    g.reusable = box.sol == nullptr;
+    // synchronisation clears this again (bp_check).  A swept grid is reusable
+    // only when it sweeps along sol itself
+    g.swept = box.sol != nullptr;
+    g.reusable = !g.swept || box.sol == c->sol.p;
+    g.seg_sol = c->sol_version;
     g.x_version = c->x_version;
     g.kind = box.kind;
     g.n = n;

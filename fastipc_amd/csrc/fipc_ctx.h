@@ -71,11 +71,14 @@ struct Grid {
     DVec<int> bstart;      // M+1 bucket offsets (bucket ends after the scatter)
     DVec<int> items;       // entries ordered by bucket
     DVec<int> qoff;        // query cell offsets (nq+1)
-    // what the grid holds, for reuse by a later operation at the same x: the
-    // x version, primitive kind, count and inflation (swept grids are never
-    // reused)
-    bool reusable = false;
-    unsigned long long x_version = 0;
+    // what the grid holds, for reuse by a later operation: the x version,
+    // primitive kind, count and inflation of a grid of boxes at x; for a grid
+    // of swept boxes (x to x + sol), the x and sol versions it sweeps between
+    // (swept): any x = x + alpha sol, 0 <= alpha <= 1, has its boxes inside
+    // the swept ones, so a query at that x may use it (the candidates are a
+    // superset and the visitors test exactly)
+    bool reusable = false, swept = false;
+    unsigned long long x_version = 0, seg_sol = 0;
     int kind = -1, n = -1;
     double infl = 0.0;
 };
@@ -189,6 +192,10 @@ struct fipc_ctx {
     // broad phase grids
     Grid grid[2];                         // [0] boundary points, [1] boundary edges
     unsigned long long x_version = 1;     // incremented by every write of x
+    // xprev holds x of version xprev_of; the x written by the last
+    // fipc_apply_sol with 0 <= alpha <= 1 (version seg_x) lies on the segment
+    // from that x along sol (version seg_sol); seg_x = 0 otherwise
+    unsigned long long xprev_of = 0, seg_x = 0, seg_base = 0, seg_sol = 0;
     // learned warm start
     DVec<double> dprev;    // previous step's correction x_{n} - xTilde_{n-1} (0 on Dirichlet vertices)
     DVec<float> wsX;       // features, nv x 16

@@ -1549,6 +1549,7 @@ extern "C" int fipc_save_prev(fipc_ctx* c)
 {
     if (!c || !c->nv) return FIPC_ERR_STATE;
     HIPCHK(hipMemcpyAsync(c->xprev.p, c->x.p, (size_t)c->nv * 24, hipMemcpyDeviceToDevice, c->s));
+    c->xprev_of = c->x_version;
     return FIPC_OK;
 }
 extern "C" int fipc_apply_sol(fipc_ctx* c, double alpha)
@@ -1557,6 +1558,16 @@ extern "C" int fipc_apply_sol(fipc_ctx* c, double alpha)
     k_apply<<<nblk(c->nv * 3), TPB, 0, c->s>>>(c->nv * 3, c->xprev.p, c->sol.p, c->x.p, alpha);
     HIPCHK(hipGetLastError());
     c->x_version++;
+    // x = xprev + alpha sol with 0 <= alpha <= 1 lies between xprev and
+    // xprev + sol coordinate by coordinate (fl(alpha s) is no larger than s
+    // and rounding is monotone), inside any box swept along sol from xprev
+    if (alpha >= 0.0 && alpha <= 1.0) {
+        c->seg_x = c->x_version;
+        c->seg_base = c->xprev_of;
+        c->seg_sol = c->sol_version;
+    } else {
+        c->seg_x = 0;
+    }
     return FIPC_OK;
 }
 extern "C" int fipc_residual(fipc_ctx* c, double* res)
@@ -1778,7 +1789,7 @@ __global__ void k_add(int n, const double* a, double* b)
 }
 extern "C" int fipc_spmv(fipc_ctx* c, const double* x, double* y)
 {
-    if (!c || !c->nv) return FIPC_ERR_STATE;
+    if (!c || !c->nv || c->cv_version != c->cons_version) return FIPC_ERR_STATE;
     size_t n = (size_t)c->nv * 24;
     HIPCHK(hipMemcpyAsync(c->pp.p, x, n, hipMemcpyHostToDevice, c->s));
     HIPCHK(hipMemsetAsync(c->qc.p, 0, n, c->s));
@@ -1951,7 +1962,7 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
 // Dirichlet increments
 extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double* relres)
 {
-    if (!c || !c->nv) return FIPC_ERR_STATE;
+    if (!c || !c->nv || c->cv_version != c->cons_version) return FIPC_ERR_STATE;
     int rc = solve_rounds(c, 0, tol, maxit, iters, relres);
     c->solve_version = c->sys_version;
     return rc;
@@ -1960,7 +1971,8 @@ extern "C" int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double
 // the tighter tolerance tol (relative to the same |r0|)
 extern "C" int fipc_solve_refine(fipc_ctx* c, double tol, int maxit, int* iters, double* relres)
 {
-    if (!c || !c->nv || c->solve_version != c->sys_version) return FIPC_ERR_STATE;
+    if (!c || !c->nv || c->solve_version != c->sys_version || c->cv_version != c->cons_version)
+        return FIPC_ERR_STATE;
     return solve_rounds(c, 1, tol, maxit, iters, relres);
 }
 

@@ -191,7 +191,10 @@ int fipc_get_constraints(fipc_ctx* c, int type, int* rows_out);
 /* output_current_minimal_distance (NEWTON_IPC.py:815-839) */
 int fipc_min_distance(fipc_ctx* c, double* d2min);
 /* compute_hessian_and_gradient (NEWTON_IPC.py:736-747): precision 0 = FP64,
- * 1 = FP32 Hessian storage (contact rows closer than 1e-2 dHat stay FP64) */
+ * 1 = FP32 Hessian storage (elastic blocks summed in FP64 and stored in FP32;
+ * contact rows in FP32 too except those closer than 1e-2 dHat).  The contact
+ * rows' gradient and diagonal blocks are summed per vertex in a fixed order
+ * (no atomics), so the result is the same bits on every run. */
 int fipc_assemble(fipc_ctx* c, int precision);
 /* solve_system (NEWTON_IPC.py:750-777) with block-Jacobi PCG instead of
  * CHOLMOD: relative residual tolerance tol, at most maxit iterations */
@@ -231,7 +234,10 @@ int fipc_get_bsr(fipc_ctx* c, int* nnzb, int* rowptr, int* col, double* val);
 /* blocks the device stores (upper triangle incl. diagonal) and the lower
  * blocks the SpMV reads transposed through (block, column) index pairs */
 int fipc_bsr_storage(fipc_ctx* c, int* nupper, int* nlower);
-/* y = A x with the assembled Hessian (elastic BSR + contact rows), Dirichlet rows zeroed */
+/* y = A x with the assembled Hessian (elastic BSR + contact rows), Dirichlet rows zeroed.
+ * FIPC_ERR_STATE if constraints were found again since the last fipc_assemble
+ * (the product would mix old Hessians with new rows); the same holds for
+ * fipc_solve and fipc_solve_refine. */
 int fipc_spmv(fipc_ctx* c, const double* x, double* y);
 /* timing of the hot kernels with HIP events on the context stream, recorded
  * without synchronisation while enabled and folded when read.  Classes:

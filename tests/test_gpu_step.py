@@ -171,3 +171,26 @@ def test_step_converged_per_oracle(scene_fn, steps):
         o.assemble()
         o.solve_system(f * sc.dt)
         assert o.output_residual() < sc.newton_tol * 1.5, (f, o.output_residual(), sc.newton_tol)
+
+
+def test_products_need_the_current_assembly():
+    """A product or solve after constraints were found again, without an
+    assembly, is refused (FIPC_ERR_STATE): its contact part would pair the
+    last assembly's Hessians with the new rows.  After the assembly both
+    work again."""
+    sc = S.tiny_pair()
+    g = NewtonIPC(sc)
+    g.compute_xn_and_xTilde(0.0)
+    g.find_constraints()
+    g.compute_hessian_and_gradient()
+    g.solve_system()
+    g.find_constraints()
+    with pytest.raises(_lib.FipcError) as e:
+        g.solve_system()
+    assert e.value.code == 6
+    v = np.zeros((sc.n_verts, 3))
+    y = np.zeros((sc.n_verts, 3))
+    assert g.so.fipc_spmv(g.ctx, _lib.dptr(v), _lib.dptr(y)) == 6
+    g.compute_hessian_and_gradient()
+    it, rel = g.solve_system()
+    assert rel <= g.pcg_tol * 1.0001
