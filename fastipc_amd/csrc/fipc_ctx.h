@@ -121,6 +121,7 @@ struct fipc_ctx {
     DVec<double> tetH, tetg;
     DVec<float> tetHf;     // the records in FP32 mode
     DVec<int> cpos, bcptr, vcptr, vcent;
+    DVec<int> gorder;      // the gather's block order: grouped by run length
     int precision = 0;     // 0 = FP64 Hessian, 1 = FP32 Hessian / SpMV
     // contacts
     int64_t cap[NTYPES] = {0};
@@ -157,6 +158,10 @@ struct fipc_ctx {
     // FP32 Hessian: the refinement rounds iterate on FP32 copies of the
     // preconditioner and of r, z, p, q (qc and the solution stay FP64)
     DVec<float> Minvf, rf, zf, pf, qf;
+    // single-reduction PCG: s = A p kept by recurrence (FP64 / FP32 rounds),
+    // and the partials of r.u and r.r of two consecutive iterations
+    DVec<double> cgs, part_g;
+    DVec<float> cgsf;
     // prescribed rows of the step (fipc_begin_step_rows)
     DVec<int> trow;
     DVec<double> tval;

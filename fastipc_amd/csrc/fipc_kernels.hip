@@ -235,17 +235,21 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) k
 constexpr int GATHER_BPW = 7;                       // blocks per wavefront
 constexpr int GATHER_BPG = GATHER_BPW * (TPB / 64); // blocks per workgroup
 static inline int gather_grid(long nnzb) { return (int)((nnzb + GATHER_BPG - 1) / GATHER_BPG); }
+// The blocks are taken in the order gorder (grouped by run length), so the
+// blocks of a wavefront loop equally long.
 template <class R>
-__global__ void __launch_bounds__(TPB) k_gather_blocks(int nnzb, const int* __restrict__ bcptr,
-                                                       const R* __restrict__ tetH, const int* __restrict__ brow,
-                                                       const int* __restrict__ col, const double* __restrict__ m,
-                                                       double* __restrict__ val, float* __restrict__ valf)
+__global__ void __launch_bounds__(TPB) k_gather_blocks(int nnzb, const int* __restrict__ gorder,
+                                                       const int* __restrict__ bcptr, const R* __restrict__ tetH,
+                                                       const int* __restrict__ brow, const int* __restrict__ col,
+                                                       const double* __restrict__ m, double* __restrict__ val,
+                                                       float* __restrict__ valf)
 {
     const int lane = threadIdx.x & 63;
     if (lane >= 9 * GATHER_BPW) return;
-    const long b = ((long)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)) * GATHER_BPW + lane / 9;
+    const long slot = ((long)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)) * GATHER_BPW + lane / 9;
     const int e = lane % 9;
-    if (b >= nnzb) return;
+    if (slot >= nnzb) return;
+    const int b = gorder[slot];
     double acc = 0.0;
     const int t1 = bcptr[b + 1];
     // unrolled so that several records are in flight (same summation order)
@@ -956,18 +960,20 @@ __global__ void k_pcg_retest(const double* scal, int* state, double tol2)
         state[1] = 0;
     }
 }
-// scal[0|1] = r.z of the current / next iteration (by parity), scal[2] =
-// |r0|^2, scal[4] = |r|^2, scal[5] = p.q; state[0] = iteration at which the
-// test passed (done_at), state[1] = iterations run.  Both update kernels run
-// a grid-stride loop over vertices on at most PCG_MAXB blocks, so the
-// partials that the next kernel reduces stay few.
-// iterations per host convergence check after the first chunk, and how many
-// iterations short of the hint the first chunk stops
+// scal[0] = r.z of a round's start, scal[2] = |r0|^2, scal[4] = |r|^2 of
+// the last tested residual, scal[5] = u.A u of the last iteration,
+// scal[6|7] and scal[8|9] = gamma and alpha of an iteration (by parity);
+// state[0] = iteration at which the test passed (done_at), state[1] =
+// iterations run.  The update kernel runs a grid-stride loop over vertices on
+// at most PCG_MAXB blocks, so the partials that the next kernel reduces stay
+// few.
 // the FP32 iterations stop at this fraction of the tolerance, so that the
 // FP64 residual that follows usually meets it
 #ifndef PCG_FP32_INNER
 #define PCG_FP32_INNER 1.0
 #endif
+// iterations per host convergence check after the first chunk, and how many
+// iterations short of the hint the first chunk stops
 #ifndef PCG_CHUNK
 #define PCG_CHUNK 8
 #endif
@@ -978,33 +984,69 @@ __global__ void k_pcg_retest(const double* scal, int* state, double tol2)
 #define PCG_MAXB 512
 #endif
 static inline int pcg_grid(int nv) { return std::min(nblk(nv), PCG_MAXB); }
-// alpha = rz / pq (pq from the SpMV partials); x += alpha p; r -= alpha q;
-// z = Minv r; partials of r.z and r.r.  V / M: storage types of the
-// iteration vectors and of the preconditioner blocks (the arithmetic is FP64;
-// x is always FP64)
+// One iteration of PCG in the single-reduction form of Chronopoulos and Gear:
+// the product of this iteration is w = A u (u = Minv r; w = q + qc) with the
+// partials of u.w (delta); the partials of gamma = r.u and of r.r were
+// written by the previous iteration into part_g[(it + 1) & 1] (it = 0: gamma
+// is scal[0] from the round's start, whose residual its own test covered).
+//   beta = gamma / gamma_prev, alpha = gamma / (delta - beta gamma / alpha_prev)
+//   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = Minv r
+// and the partials of the new r.u and r.r go to part_g[it & 1].  The test of
+// an iteration's residual runs at the start of the next one (every block
+// decides it from the same partials; block 0 records it): on success
+// done_at = it - 1 and nothing more is written.  gamma and alpha of an
+// iteration are kept by parity in scal[6 + (it & 1)] and scal[8 + (it & 1)],
+// so no block reads a value another block of the same launch writes.  One
+// launch replaces the two update kernels of the textbook form (whose second
+// waits for the r.z reduction of the first).
 template <class V, class M>
-__global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const unsigned char* __restrict__ dmask,
-                                                  const M* __restrict__ Minv, const V* __restrict__ p,
-                                                  const V* __restrict__ q, double* __restrict__ qc,
-                                                  double* __restrict__ x, V* __restrict__ r, V* __restrict__ z,
-                                                  const double* part_pq, int npq, double* part_rz, double* part_rr,
-                                                  const int* state, int it)
+__global__ void __launch_bounds__(TPB) k_pcg_cg(int nv, double* scal, const unsigned char* __restrict__ dmask,
+                                                const M* __restrict__ Minv, V* __restrict__ u,
+                                                const V* __restrict__ w, double* __restrict__ qc,
+                                                V* __restrict__ p, V* __restrict__ s, double* __restrict__ x,
+                                                V* __restrict__ r, const double* part_uw, int nuw, double* part_g,
+                                                int npg, int* state, int it, double tol2)
 {
     if (it > state[0]) return;
-    double pq = reduce_partials(part_pq, npq);
-    double alpha = scal[it & 1] / pq;
-    if (blockIdx.x == 0 && threadIdx.x == 0) scal[5] = pq;
-    double rz = 0.0, rr = 0.0;
+    const int cur = it & 1, prv = cur ^ 1;
+    double gamma, rr;
+    if (it == 0) {
+        gamma = scal[0];
+        rr = scal[4];
+    } else {
+        reduce_partials2(part_g + 2 * PCG_MAXB * prv, part_g + 2 * PCG_MAXB * prv + PCG_MAXB, npg, gamma, rr);
+        if (rr <= tol2 * scal[2] || !(rr == rr)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                state[0] = it - 1;
+                state[1] = it;
+                scal[4] = rr;
+            }
+            return;
+        }
+    }
+    const double delta = reduce_partials(part_uw, nuw);
+    const double beta = it == 0 ? 0.0 : gamma / scal[6 + prv];
+    const double alpha = it == 0 ? gamma / delta : gamma / (delta - beta * gamma / scal[8 + prv]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[6 + cur] = gamma;
+        scal[8 + cur] = alpha;
+        scal[4] = rr;
+        scal[5] = delta;
+        state[1] = it + 1;
+    }
+    double ru = 0.0, rn2 = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
         if (dmask[i]) continue;
         // every load of the vertex is issued before the first store
         const int k = 3 * i;
-        double pv[3], qv[3], qcv[3], xv[3], rv[3], Mi[9];
+        double uv[3], wv[3], qcv[3], pv[3], sv[3], xv[3], rv[3], Mi[9];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            pv[d] = (double)p[k + d];
-            qv[d] = (double)q[k + d];
+            uv[d] = (double)u[k + d];
+            wv[d] = (double)w[k + d];
             qcv[d] = qc[k + d];
+            pv[d] = it == 0 ? 0.0 : (double)p[k + d];
+            sv[d] = it == 0 ? 0.0 : (double)s[k + d];
             xv[d] = x[k + d];
             rv[d] = (double)r[k + d];
         }
@@ -1013,48 +1055,45 @@ __global__ void __launch_bounds__(TPB) k_pcg_upd1(int nv, double* scal, const un
         double ri[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            x[k + d] = xv[d] + alpha * pv[d];
             // qc is non-zero only at contact vertices: cleared here, where
             // it is consumed, for the next product
             if (qcv[d] != 0.0) qc[k + d] = 0.0;
-            V rn = (V)(rv[d] - alpha * (qv[d] + qcv[d]));
-            r[k + d] = rn;
-            ri[d] = (double)rn;
+            const V pn = (V)(uv[d] + beta * pv[d]);
+            const V sn = (V)((wv[d] + qcv[d]) + beta * sv[d]);
+            p[k + d] = pn;
+            s[k + d] = sn;
+            x[k + d] = xv[d] + alpha * (double)pn;
+            const V rnv = (V)(rv[d] - alpha * (double)sn);
+            r[k + d] = rnv;
+            ri[d] = (double)rnv;
         }
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            V zv = (V)(Mi[3 * d] * ri[0] + Mi[3 * d + 1] * ri[1] + Mi[3 * d + 2] * ri[2]);
-            z[k + d] = zv;
-            rz += ri[d] * (double)zv;
-            rr += ri[d] * ri[d];
+            const V un = (V)(Mi[3 * d] * ri[0] + Mi[3 * d + 1] * ri[1] + Mi[3 * d + 2] * ri[2]);
+            u[k + d] = un;
+            ru += ri[d] * (double)un;
+            rn2 += ri[d] * ri[d];
         }
     }
-    rz = block_sum(rz);
-    rr = block_sum(rr);
+    ru = block_sum(ru);
+    rn2 = block_sum(rn2);
     if (threadIdx.x == 0) {
-        part_rz[blockIdx.x] = rz;
-        part_rr[blockIdx.x] = rr;
+        part_g[2 * PCG_MAXB * cur + blockIdx.x] = ru;
+        part_g[2 * PCG_MAXB * cur + PCG_MAXB + blockIdx.x] = rn2;
     }
 }
-// reduces r.z and r.r, runs the convergence test (block 0 records it),
-// beta = rz_new / rz; p = z + beta p
-template <class V>
-__global__ void __launch_bounds__(TPB) k_pcg_upd2(int nv, double* scal, const V* __restrict__ z, V* __restrict__ p,
-                                                  const double* part_rz, const double* part_rr, int n1,
-                                                  int* state, int it, double tol2)
+// the test of the last iteration's residual when the iteration budget ended
+// before the next iteration could run it
+__global__ void k_pcg_cg_last(double* scal, const double* part_g, int npg, int* state, int it, double tol2)
 {
     if (it > state[0]) return;
-    double rz_new, rr;
-    reduce_partials2(part_rz, part_rr, n1, rz_new, rr);
-    double beta = rz_new / scal[it & 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        scal[(it + 1) & 1] = rz_new;
+    const int prv = (it & 1) ^ 1;
+    double gamma, rr;
+    reduce_partials2(part_g + 2 * PCG_MAXB * prv, part_g + 2 * PCG_MAXB * prv + PCG_MAXB, npg, gamma, rr);
+    if (threadIdx.x == 0) {
         scal[4] = rr;
-        state[1] = it + 1;
-        if (rr <= tol2 * scal[2] || !(rr == rr)) state[0] = it;
+        if (rr <= tol2 * scal[2] || !(rr == rr)) state[0] = it - 1;
     }
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nv; i += gridDim.x * blockDim.x)
-        p[i] = (V)((double)z[i] + beta * (double)p[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1267,7 +1306,7 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->x); release(c->x0); release(c->xn); release(c->xt); release(c->xprev); release(c->v);
     release(c->sol); release(c->rhs); release(c->target);
     release(c->rowptr); release(c->col); release(c->diag); release(c->val); release(c->valf);
-    release(c->brow); release(c->tetH); release(c->tetHf); release(c->tetg); release(c->bcptr); release(c->cpos); release(c->vcptr);
+    release(c->brow); release(c->tetH); release(c->tetHf); release(c->tetg); release(c->bcptr); release(c->gorder); release(c->cpos); release(c->vcptr);
     release(c->lptr); release(c->lent);
     release(c->vcent);
     for (int t = 0; t < NTYPES; t++) { release(c->rows[t]); release(c->cH[t]); release(c->cHf[t]); release(c->c64[t]); }
@@ -1276,6 +1315,7 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->keys_pp); release(c->keys_pe); release(c->keys_tmp); release(c->counters); release(c->cub_tmp); release(c->cub_tmp2); release(c->rows_tmp);
     release(c->nuniq); release(c->cdiag); release(c->Minv); release(c->r); release(c->z); release(c->pp); release(c->q); release(c->qc);
     release(c->Minvf); release(c->rf); release(c->zf); release(c->pf); release(c->qf);
+    release(c->cgs); release(c->cgsf); release(c->part_g);
     release(c->trow); release(c->tval);
     release(c->part_a); release(c->part_b); release(c->part_c); release(c->scal); release(c->pcg_state);
     release(c->x0flag);
@@ -1360,6 +1400,20 @@ static int build_pattern(fipc_ctx* c, int nv, int nt, const int* tets)
     std::vector<int> vcent((size_t)std::max(nt, 1) * 4), vfill(vcnt.begin(), vcnt.end() - 1);
     for (int e = 0; e < nt; e++)
         for (int a = 0; a < 4; a++) vcent[vfill[tets[4 * e + a]]++] = a * nt + e;
+    // gather order: the blocks grouped by run length (stably, so each group
+    // keeps index order), so that the seven blocks of a wavefront run loops
+    // of equal length (a row's diagonal block gathers about 20 records, its
+    // off-diagonal ones 1 to 6)
+    std::vector<int> gorder(std::max(nnzb, 1));
+    {
+        int maxlen = 0;
+        for (int b = 0; b < nnzb; b++) maxlen = std::max(maxlen, bcnt[b + 1] - bcnt[b]);
+        std::vector<int> lstart(maxlen + 2, 0);
+        for (int b = 0; b < nnzb; b++) lstart[bcnt[b + 1] - bcnt[b] + 1]++;
+        for (int l = 0; l <= maxlen; l++) lstart[l + 1] += lstart[l];
+        for (int b = 0; b < nnzb; b++) gorder[lstart[bcnt[b + 1] - bcnt[b]]++] = b;
+    }
+    STCHK(upload(c->gorder, gorder.data(), gorder.size(), c->s));
     STCHK(upload(c->bcptr, bcnt.data(), bcnt.size(), c->s));
     STCHK(upload(c->cpos, cpos.data(), cpos.size(), c->s));
     STCHK(upload(c->brow, brow.data(), brow.size(), c->s));
@@ -1420,7 +1474,7 @@ extern "C" int fipc_upload_mesh(fipc_ctx* c, int nv, const double* x0, const dou
     // (into part_a) at most 1024; the SpMV grid has at most SPMV_MAXB blocks
     STCHK(ensure(c->part_a, std::max(nblk(nmax), 1024) + 64)); STCHK(ensure(c->part_b, nblk(nmax) + 64));
     STCHK(ensure(c->part_c, SPMV_MAXB + 64));
-    STCHK(ensure(c->scal, 8)); STCHK(ensure(c->pcg_state, 4));
+    STCHK(ensure(c->scal, 16)); STCHK(ensure(c->pcg_state, 4));
     STCHK(ensure(c->epart, nblk(nmax) * 4 + 64));
     STCHK(ensure(c->amin, 4)); STCHK(ensure(c->flag, 4)); STCHK(ensure(c->counters, 16)); STCHK(ensure(c->nuniq, 4));
     // broad-phase capacities start from the primitive counts and grow on demand
@@ -1716,10 +1770,10 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     STCHK(fipc_tm_end(c, TM_ELASTIC));
     STCHK(fipc_tm_begin(c, TM_GATHER));
     if (precision == 1)
-        k_gather_blocks<float><<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->tetHf.p, c->brow.p,
+        k_gather_blocks<float><<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->gorder.p, c->bcptr.p, c->tetHf.p, c->brow.p,
                                                                         c->col.p, c->m.p, nullptr, c->valf.p);
     else
-        k_gather_blocks<double><<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->bcptr.p, c->tetH.p, c->brow.p,
+        k_gather_blocks<double><<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->gorder.p, c->bcptr.p, c->tetH.p, c->brow.p,
                                                                          c->col.p, c->m.p, c->val.p, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamWaitEvent(c->s, c->ev_contact, 0));
@@ -1808,6 +1862,8 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
                      double tol2, int hint, int state[2], bool reuse, bool* init_only)
 {
     const int nv = c->nv, nbv = nblk(nv), nsp = spmv_nparts(c), nu = pcg_grid(nv), chunk = PCG_CHUNK;
+    const double tol_it = prec == 1 ? tol2 * (PCG_FP32_INNER * PCG_FP32_INNER) : tol2;
+    V* sv = sizeof(V) == 4 ? (V*)(void*)c->cgsf.p : (V*)(void*)c->cgs.p;
     int it = 0;
     state[0] = 0x7fffffff;
     state[1] = 0;
@@ -1843,19 +1899,25 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
     int first = std::max(chunk, hint - PCG_FIRST_SHORT);
     while (it < budget) {
         int end = std::min(budget, it + (it == 0 ? first : chunk));
+        // single-reduction iterations: the product of u = z, then one update
         for (; it < end; it++) {
-            STCHK(spmv<V>(c, prec, p, q, c->qc.p, c->part_c.p, c->pcg_state.p, it));
+            STCHK(spmv<V>(c, prec, z, q, c->qc.p, c->part_c.p, c->pcg_state.p, it));
             STCHK(fipc_tm_begin(c, TM_PCGVEC));
-            k_pcg_upd1<V, M><<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->dmask.p, Minv, p, q, c->qc.p, c->sol.p, r, z,
-                                                   c->part_c.p, nsp, c->part_a.p, c->part_b.p, c->pcg_state.p, it);
-            k_pcg_upd2<V><<<nu, TPB, 0, c->s>>>(nv, c->scal.p, z, p, c->part_a.p, c->part_b.p, nu, c->pcg_state.p,
-                                                it, prec == 1 ? tol2 * (PCG_FP32_INNER * PCG_FP32_INNER) : tol2);
+            k_pcg_cg<V, M><<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->dmask.p, Minv, z, q, c->qc.p, p, sv, c->sol.p, r,
+                                                 c->part_c.p, nsp, c->part_g.p, nu, c->pcg_state.p, it, tol_it);
             STCHK(fipc_tm_end(c, TM_PCGVEC));
         }
         HIPCHK(hipGetLastError());
         STCHK(d2h_async(c, state, c->pcg_state.p, 8));
         STCHK(d2h_wait(c));
         if (state[0] != 0x7fffffff) break;
+    }
+    // the budget ran out: the last iteration's residual is tested here
+    if (state[0] == 0x7fffffff && it > 0) {
+        k_pcg_cg_last<<<1, TPB, 0, c->s>>>(c->scal.p, c->part_g.p, nu, c->pcg_state.p, it, tol_it);
+        HIPCHK(hipGetLastError());
+        STCHK(d2h_async(c, state, c->pcg_state.p, 8));
+        STCHK(d2h_wait(c));
     }
     if (budget <= 0) {
         STCHK(d2h_async(c, state, c->pcg_state.p, 8));
@@ -1883,7 +1945,11 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
     if (prec == 1) {
         STCHK(ensure(c->rf, (size_t)nv * 3)); STCHK(ensure(c->zf, (size_t)nv * 3));
         STCHK(ensure(c->pf, (size_t)nv * 3)); STCHK(ensure(c->qf, (size_t)nv * 3));
+        STCHK(ensure(c->cgsf, (size_t)nv * 3));
+    } else {
+        STCHK(ensure(c->cgs, (size_t)nv * 3));
     }
+    STCHK(ensure(c->part_g, 4 * PCG_MAXB));
     const int rounds = prec == 1 ? 4 : 1;
     int total = 0, state[2] = {0, 0};
     bool converged = false, init_only = false;
