@@ -156,6 +156,7 @@ def run_gpu(args, variant):
         log(f"warmup step {i}: {time.perf_counter() - t1:.3f}s newton {st.newton_iters} pcg {st.pcg_iters} "
             f"contacts {st.counts[-1] if st.counts else None}")
     _lib.call("fipc_sync", sim.ctx)
+    phases0 = dict(sim.timers.total)
     t0 = time.perf_counter()
     pcg = []
     for i in range(args.steps):
@@ -170,6 +171,10 @@ def run_gpu(args, variant):
             raise RuntimeError("intersection after a step")
     _lib.call("fipc_sync", sim.ctx)
     wall = time.perf_counter() - t0
+    # host wall time by phase of the timed steps (NewtonIPC's phase timers,
+    # two levels deep), ms per step
+    phases = {k: round((v - phases0.get(k, 0.0)) / args.steps * 1e3, 3) for k, v in sim.timers.total.items()
+              if k.count("/") <= 2 and v > phases0.get(k, 0.0)}
     if args.traffic_probe:
         # counter-calibration run: a plain device copy of known size
         cms = C.c_double()
@@ -256,6 +261,7 @@ def run_gpu(args, variant):
         "roofline": roof,
         "kernel_classes": others,
         "kernel_times_event_steps": kt,
+        "host_phases_ms_per_step": phases,
         "setup_s": setup_s,
         "nv": sim.nv,
         "nt": len(sc.tets),
@@ -567,6 +573,7 @@ def main():
         "kernel_classes": g["kernel_classes"],
         "kernel_times": {k: {"launches": v["launches"], "us_avg": round(v["us_avg"], 2)}
                          for k, v in g["kernel_times_event_steps"].items() if v["launches"]},
+        "host_phases_ms_per_step": g["host_phases_ms_per_step"],
     }
     print(json.dumps(line))
 

@@ -275,17 +275,25 @@ def run_eval(mlp, family, seeds, steps, warmup, size, log=print, precision=0):
                 sim.step(warm_start=hook)
             _lib.call("fipc_sync", sim.ctx)
             t0 = time.perf_counter()
-            its, acc = [], 0
+            its, acc, stalled = [], 0, []
             for _ in range(steps):
                 st = sim.step(warm_start=hook)
                 its.append(st.newton_iters)
+                stalled.append(bool(st.stalled))
                 acc += int(st.warm_accepted)
                 if sim.check_collision():
                     raise RuntimeError("intersection after a step")
             _lib.call("fipc_sync", sim.ctx)
             wall = time.perf_counter() - t0
+            its_a, st_a = np.array(its), np.array(stalled)
+            # steps that converged (a stalled step ends when the line search
+            # can no longer lower the energy, however good the start was)
+            conv = its_a[~st_a]
             row[mode] = {"newton_per_step": float(np.mean(its)), "steps_per_s": steps / wall,
-                         "warm_accepted": acc}
+                         "warm_accepted": acc, "stalled_steps": int(st_a.sum()),
+                         "newton_per_converged_step": float(conv.mean()) if len(conv) else None,
+                         "newton_per_step_by_quarter": [round(float(q.mean()), 2)
+                                                        for q in np.array_split(its_a, 4) if len(q)]}
             xs[mode] = sim.x
             sim.close()
         for mode in ("extrapolation", "learned"):

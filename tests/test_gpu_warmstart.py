@@ -255,3 +255,29 @@ def test_capture_near_contacts():
         assert np.all(X[:, 9] < 1), (near.sum(), X[:, 9].max())
     print(f"{near.sum()} vertices near contacts")
     m.close()
+
+
+@pytest.mark.parametrize("family,steps", [("slab", 30), ("cubes", 60)])
+def test_shipped_weights_step_their_family(family, steps):
+    """The shipped weights (fastipc_amd/weights/ws_<family>.npz) load, carry
+    their provenance, and warm-start a held-out member of their scene family
+    (seed 1000, the small size): every step ends intersection-free and
+    converged (or stalled, see NewtonIPC.step), and the prediction is kept in
+    some steps."""
+    import os
+    from fastipc_amd import warmstart as W
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    net = W.WarmStartMLP.load(os.path.join(root, "fastipc_amd", "weights", f"ws_{family}.npz"))
+    assert net.meta.get("family") == family
+    sc = W.family_scene(family, 1000, "small")
+    sim = NewtonIPC(sc)
+    hook = W.LearnedWarmStart(net, W.scale_for(sc))
+    kept = 0
+    for k in range(steps):
+        st = sim.step(warm_start=hook)
+        assert st.converged or st.stalled, k
+        assert sim.check_collision() == 0, k
+        kept += int(st.warm_accepted)
+    print(f"{family}: warm start kept in {kept} of {steps} steps")
+    assert kept > 0
+    net.close()
