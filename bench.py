@@ -108,6 +108,9 @@ VARIANTS = {
     # search directions solved to --pcg-tol-search, the direction that passes
     # the Newton test continued to --pcg-tol before the test is repeated
     "plain+inexact": (False, None, True),
+    # the same form with the zero network's start (second-order
+    # extrapolation of the correction): what the learned residual adds
+    "fp32+extrapolation+inexact": (True, "extrapolation", True),
     "chosen+inexact": (True, "learned", True),
 }
 WEIGHTS = os.path.join(ROOT, "fastipc_amd", "weights", "ws_slab.npz")
@@ -515,7 +518,8 @@ def main():
     if g is None:
         return
     others = {}
-    extra = [] if args.no_plain else [v for v in ("plain", "plain+inexact") if v != args.mode]
+    extra = [] if args.no_plain else [v for v in ("plain", "plain+inexact", "fp32+extrapolation+inexact")
+                                      if v != args.mode]
     if args.all_variants:
         extra += [v for v in VARIANTS if v not in (args.mode, "plain")]
     for v in extra:
@@ -526,7 +530,7 @@ def main():
                      "warm_start_kept": r["warm_start_kept"]}
     full = {}
     if not args.no_full_trajectory:
-        for v in dict.fromkeys((args.mode, "plain")):
+        for v in dict.fromkeys((args.mode, "plain") + (() if args.no_plain else ("fp32+extrapolation+inexact",))):
             full[v] = run_full_trajectory(args, v)
         if "plain" in full and args.mode != "plain":
             full["speedup_vs_plain_fp64"] = round(full[args.mode]["steps_per_s"] / full["plain"]["steps_per_s"], 4)

@@ -255,7 +255,8 @@ def validation(mlp, family, seed, steps, size="bench", nsample=16384, skip=10, l
     return out
 
 
-def run_eval(mlp, family, seeds, steps, warmup, size, log=print, precision=0):
+def run_eval(mlp, family, seeds, steps, warmup, size, log=print, precision=0,
+             modes=("plain", "extrapolation", "learned")):
     """Newton iterations per step and steps/s on the same scenes for three
     starts: x_n (plain, the reference), the zero network (second-order
     extrapolation) and the trained network; final positions of the warm-started
@@ -266,7 +267,7 @@ def run_eval(mlp, family, seeds, steps, warmup, size, log=print, precision=0):
     for sd in seeds:
         row = {"seed": sd}
         xs = {}
-        for mode in ("plain", "extrapolation", "learned"):
+        for mode in modes:
             sc = family_scene(family, sd, size)
             sim = NewtonIPC(sc, precision=precision)
             net = {"plain": None, "extrapolation": zero, "learned": mlp}[mode]
@@ -297,7 +298,8 @@ def run_eval(mlp, family, seeds, steps, warmup, size, log=print, precision=0):
             xs[mode] = sim.x
             sim.close()
         for mode in ("extrapolation", "learned"):
-            row[mode]["max_position_difference_over_bbox"] = float(np.abs(xs[mode] - xs["plain"]).max() / sc.bbox_diag)
+            if mode in xs and "plain" in xs:
+                row[mode]["max_position_difference_over_bbox"] = float(np.abs(xs[mode] - xs["plain"]).max() / sc.bbox_diag)
         log(json.dumps(row))
         out.append(row)
     zero.close()
@@ -332,6 +334,8 @@ def main(argv=None):
     ev.add_argument("--steps", type=int, default=20)
     ev.add_argument("--warmup", type=int, default=10)
     ev.add_argument("--precision", type=int, default=0)
+    ev.add_argument("--modes", default="plain,extrapolation,learned",
+                    help="comma-separated starts to run: plain, extrapolation, learned")
     a = ap.parse_args(argv)
 
     def seeds(spec):
@@ -373,7 +377,8 @@ def main(argv=None):
         log(json.dumps(meta))
     else:
         mlp = WarmStartMLP.load(a.weights)
-        run_eval(mlp, a.family, seeds(a.seeds), a.steps, a.warmup, a.size, log=log, precision=a.precision)
+        run_eval(mlp, a.family, seeds(a.seeds), a.steps, a.warmup, a.size, log=log, precision=a.precision,
+                 modes=tuple(m for m in a.modes.split(",") if m))
 
 
 if __name__ == "__main__":
