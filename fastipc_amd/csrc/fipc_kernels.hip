@@ -210,7 +210,9 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) k
     nh_factor(F, lae, mue, f);
     // block (a, c), a <= c, belongs to the upper block (min, max) of the global
     // vertices: transposed when the global order is reversed
-    tet_hessian_blocks(f, Bm, sc, [&](int a, int c, const double* blk) {
+    // FP32 records are computed in FP32 arithmetic from the FP64 factors
+    using T = typename std::conditional<sizeof(R) == 4, float, double>::type;
+    tet_hessian_blocks_t<T>(f, Bm, sc, [&](int a, int c, const T* blk) {
         R* o = tetH + (size_t)cpos[(size_t)e * 10 + uslot(a, c)] * 9;
         if (t[a] <= t[c]) {
 #pragma unroll

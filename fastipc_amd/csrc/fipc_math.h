@@ -681,45 +681,63 @@ FI void nh_dPdF(const double* F, double la, double mu, double* D)
         }
 }
 
-// 12x12 tet Hessian as Z M Z^T (see fipc_math.h NHFactor), accumulated per
-// 3x3 block.  emit(a, c, blk) receives the block of vertex pair (a, c), c >= a.
-template <class Emit>
-FI void tet_hessian_blocks(const NHFactor& f, const double (&Bm)[4][3], double sc, Emit emit)
+// 12x12 tet Hessian as Z M Z^T (see NHFactor), accumulated per 3x3 block in
+// the arithmetic type T (double, or float where the blocks are stored in
+// FP32 anyway: the factors U, V, S, B are computed in FP64 and rounded once).
+// emit(a, c, blk) receives the block of vertex pair (a, c), c >= a, as T[9].
+template <class T, class Emit>
+FI void tet_hessian_blocks_t(const NHFactor& f, const double (&Bm_)[4][3], double sc_, Emit emit)
 {
     // derivative-only arithmetic: contraction into FMA is allowed here
 #pragma clang fp contract(fast)
-    // bv[a][b] = sum_j Bm[a][j] V[j][b]
-    double bv[4][3];
+    T U[9], V[9], S[3][3], Bk[3][4], Bm[4][3];
+#pragma unroll
+    for (int k = 0; k < 9; k++) { U[k] = (T)f.U[k]; V[k] = (T)f.V[k]; }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) S[i][j] = (T)f.S[i][j];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) Bk[k][j] = (T)f.B[k][j];
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 3; b++) bv[a][b] = Bm[a][0] * f.V[0 * 3 + b] + Bm[a][1] * f.V[1 * 3 + b] + Bm[a][2] * f.V[2 * 3 + b];
+        for (int j = 0; j < 3; j++) Bm[a][j] = (T)Bm_[a][j];
+    const T sc = (T)sc_;
+    // bv[a][b] = sum_j Bm[a][j] V[j][b]
+    T bv[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) bv[a][b] = Bm[a][0] * V[0 * 3 + b] + Bm[a][1] * V[1 * 3 + b] + Bm[a][2] * V[2 * 3 + b];
     // M in the (ka,kb) pair basis: diagonal pairs via S, off pairs via B blocks
     // B block k couples idx (i,j) and (j,i) for (i,j) = (0,1), (1,2), (2,0)
     const int PI[3] = {0, 1, 2}, PJ[3] = {1, 2, 0};
 #pragma unroll
     for (int a = 0; a < 4; a++) {
         // Y[i][l] for l = ka*3+kb: sum_k z_k[(a,i)] M[k][l]
-        double Y[3][9];
+        T Y[3][9];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
 #pragma unroll
-            for (int l = 0; l < 9; l++) Y[i][l] = 0.0;
+            for (int l = 0; l < 9; l++) Y[i][l] = (T)0;
 #pragma unroll
             for (int be = 0; be < 3; be++) {
-                double s = 0.0;
+                T s = (T)0;
 #pragma unroll
-                for (int al = 0; al < 3; al++) s += bv[a][al] * f.U[i * 3 + al] * f.S[al][be];
+                for (int al = 0; al < 3; al++) s += bv[a][al] * U[i * 3 + al] * S[al][be];
                 Y[i][be * 4] = s * sc;
             }
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 const int pi = PI[k], pj = PJ[k];
                 // z_(pi,pj)[(a,i)] = bv[a][pj] U[i][pi]; z_(pj,pi)[(a,i)] = bv[a][pi] U[i][pj]
-                double zij = bv[a][pj] * f.U[i * 3 + pi], zji = bv[a][pi] * f.U[i * 3 + pj];
+                T zij = bv[a][pj] * U[i * 3 + pi], zji = bv[a][pi] * U[i * 3 + pj];
                 // block k on the basis ((pi,pj), (pj,pi)), row-major (neo_hookean.py:112-123;
                 // for k = 2 the basis is (6, 2) in the reference's M indexing)
-                const double B00 = f.B[k][0], B01 = f.B[k][1], B10 = f.B[k][2], B11 = f.B[k][3];
+                const T B00 = Bk[k][0], B01 = Bk[k][1], B10 = Bk[k][2], B11 = Bk[k][3];
                 Y[i][pi * 3 + pj] = (zij * B00 + zji * B10) * sc;
                 Y[i][pj * 3 + pi] = (zij * B01 + zji * B11) * sc;
             }
@@ -727,22 +745,29 @@ FI void tet_hessian_blocks(const NHFactor& f, const double (&Bm)[4][3], double s
 #pragma unroll
         for (int c = a; c < 4; c++) {
             // z_l[(c,j)] = bv[c][lb] U[j][la] for l = la*3+lb, so
-            // blk[i][j] = sum_la U[j][la] T[i][la], T[i][la] = sum_lb Y[i][la*3+lb] bv[c][lb]
-            double T[3][3];
+            // blk[i][j] = sum_la U[j][la] Tm[i][la], Tm[i][la] = sum_lb Y[i][la*3+lb] bv[c][lb]
+            T Tm[3][3];
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int la = 0; la < 3; la++)
-                    T[i][la] = Y[i][la * 3] * bv[c][0] + Y[i][la * 3 + 1] * bv[c][1] + Y[i][la * 3 + 2] * bv[c][2];
-            double blk[9];
+                    Tm[i][la] = Y[i][la * 3] * bv[c][0] + Y[i][la * 3 + 1] * bv[c][1] + Y[i][la * 3 + 2] * bv[c][2];
+            T blk[9];
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int j = 0; j < 3; j++)
-                    blk[i * 3 + j] = T[i][0] * f.U[j * 3] + T[i][1] * f.U[j * 3 + 1] + T[i][2] * f.U[j * 3 + 2];
+                    blk[i * 3 + j] = Tm[i][0] * U[j * 3] + Tm[i][1] * U[j * 3 + 1] + Tm[i][2] * U[j * 3 + 2];
             emit(a, c, blk);
         }
     }
+}
+
+// the FP64 form (host drop-ins, FP64 mode)
+template <class Emit>
+FI void tet_hessian_blocks(const NHFactor& f, const double (&Bm)[4][3], double sc, Emit emit)
+{
+    tet_hessian_blocks_t<double>(f, Bm, sc, emit);
 }
 
 // ---------------------------------------------------------------------------
