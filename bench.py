@@ -52,6 +52,7 @@ HBM_PEAK_GBS = 8000.0
 TIMER_NAMES = ("spmv", "elastic_tet_pass", "ccd_query", "contact_query", "energy", "contact_assembly",
                "pcg_vector", "grid_build", "assembly_gather", "spmv_residual", "ccd_narrow")
 FP64_VALU_PEAK_TFLOPS = 78.6
+FP32_VALU_PEAK_TFLOPS = 157.3
 
 
 def log(msg):
@@ -229,10 +230,17 @@ def run_gpu(args, variant):
     # counters over its live event-timed duration
     flops = valu_flops()
     if flops.get("k_elastic") and kt["elastic_tet_pass"]["us_avg"]:
+        # the pass runs FP64 (SVD, projections, gradient) and, in FP32 mode,
+        # FP32 (the 12x12 contraction): its VALU fraction is the time each
+        # share would take at its own peak, over the measured duration
+        dur = kt["elastic_tet_pass"]["us_avg"] * 1e-6
         f = flops["k_elastic"]
-        tf = f / (kt["elastic_tet_pass"]["us_avg"] * 1e-6) / 1e12
+        f32 = flops.get("k_elastic_fp32", 0) if precision == 1 else 0
+        tf = f / dur / 1e12
+        frac = (f / (FP64_VALU_PEAK_TFLOPS * 1e12) + f32 / (FP32_VALU_PEAK_TFLOPS * 1e12)) / dur
         others["elastic_assembly"].update({"tet_pass_fp64_flops_per_launch": f, "tet_pass_fp64_tflops": round(tf, 2),
-                                           "tet_pass_valu_frac": round(tf / FP64_VALU_PEAK_TFLOPS, 4)})
+                                           "tet_pass_fp32_flops_per_launch": f32,
+                                           "tet_pass_valu_frac": round(frac, 4)})
     # CCD pass (point-triangle and edge-edge candidate queries on the swept
     # boxes, then the narrow phases: two timed query and two narrow-phase
     # launches per pass); its compulsory unique bytes are x and the direction
@@ -276,17 +284,19 @@ def run_gpu(args, variant):
 
 
 def valu_flops():
-    """FP64 operations per launch of the FP64-VALU-bound kernels, from the
-    committed rocprofv3 counter summary (lines 'fp64_flops_per_launch:
-    <kernel>=<N>'): 64 lanes x (ADD + MUL + TRANS + 2 FMA) FP64 VALU
-    instructions."""
+    """FP64 (and FP32, keyed <kernel>_fp32) operations per launch of the
+    VALU-heavy kernels, from the committed rocprofv3 counter summary (lines
+    'fp64_flops_per_launch: <kernel>=<N>' and 'fp32_flops_per_launch: ...'):
+    64 lanes x (ADD + MUL + TRANS + 2 FMA) VALU instructions."""
     import re
-    path = os.path.join(ROOT, "profiles", "valu_fp64_compressed_slab.md")
+    path = os.path.join(ROOT, "profiles", "valu_compressed_slab.md")
     try:
         txt = open(path).read()
     except OSError:
         return {}
-    return {k: int(v) for k, v in re.findall(r"fp64_flops_per_launch:\s*(\w+)=(\d+)", txt)}
+    out = {k: int(v) for k, v in re.findall(r"fp64_flops_per_launch:\s*(\w+)=(\d+)", txt)}
+    out.update({k + "_fp32": int(v) for k, v in re.findall(r"fp32_flops_per_launch:\s*(\w+)=(\d+)", txt)})
+    return out
 
 
 def ccd_narrow_probe(sim, flops):
