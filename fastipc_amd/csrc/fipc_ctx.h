@@ -121,7 +121,7 @@ struct fipc_ctx {
     DVec<double> tetH, tetg;
     DVec<float> tetHf;     // the records in FP32 mode
     DVec<int> cpos, bcptr, vcptr, vcent;
-    DVec<int> gorder;      // the gather's block order: grouped by run length
+    DVec<int> gorder;      // the gather's block order: diagonal blocks first
     int precision = 0;     // 0 = FP64 Hessian, 1 = FP32 Hessian / SpMV
     // contacts
     int64_t cap[NTYPES] = {0};

@@ -237,8 +237,8 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) k
 constexpr int GATHER_BPW = 7;                       // blocks per wavefront
 constexpr int GATHER_BPG = GATHER_BPW * (TPB / 64); // blocks per workgroup
 static inline int gather_grid(long nnzb) { return (int)((nnzb + GATHER_BPG - 1) / GATHER_BPG); }
-// The blocks are taken in the order gorder (grouped by run length), so the
-// blocks of a wavefront loop equally long.
+// The blocks are taken in the order gorder (diagonal blocks first), so the
+// blocks of a wavefront loop about equally long.
 template <class R>
 __global__ void __launch_bounds__(TPB) k_gather_blocks(int nnzb, const int* __restrict__ gorder,
                                                        const int* __restrict__ bcptr, const R* __restrict__ tetH,
@@ -1402,18 +1402,16 @@ static int build_pattern(fipc_ctx* c, int nv, int nt, const int* tets)
     std::vector<int> vcent((size_t)std::max(nt, 1) * 4), vfill(vcnt.begin(), vcnt.end() - 1);
     for (int e = 0; e < nt; e++)
         for (int a = 0; a < 4; a++) vcent[vfill[tets[4 * e + a]]++] = a * nt + e;
-    // gather order: the blocks grouped by run length (stably, so each group
-    // keeps index order), so that the seven blocks of a wavefront run loops
-    // of equal length (a row's diagonal block gathers about 20 records, its
-    // off-diagonal ones 1 to 6)
+    // gather order: the rows' diagonal blocks first (each gathers about 20
+    // records), then the off-diagonal blocks (1 to 6 records) in index order,
+    // so that the seven blocks of a wavefront loop about equally long and the
+    // off-diagonal runs are still read in address order
     std::vector<int> gorder(std::max(nnzb, 1));
     {
-        int maxlen = 0;
-        for (int b = 0; b < nnzb; b++) maxlen = std::max(maxlen, bcnt[b + 1] - bcnt[b]);
-        std::vector<int> lstart(maxlen + 2, 0);
-        for (int b = 0; b < nnzb; b++) lstart[bcnt[b + 1] - bcnt[b] + 1]++;
-        for (int l = 0; l <= maxlen; l++) lstart[l + 1] += lstart[l];
-        for (int b = 0; b < nnzb; b++) gorder[lstart[bcnt[b + 1] - bcnt[b]]++] = b;
+        int k = 0;
+        for (int i = 0; i < nv; i++) gorder[k++] = rowptr[i];  // the diagonal block leads its row
+        for (int i = 0; i < nv; i++)
+            for (int b = rowptr[i] + 1; b < rowptr[i + 1]; b++) gorder[k++] = b;
     }
     STCHK(upload(c->gorder, gorder.data(), gorder.size(), c->s));
     STCHK(upload(c->bcptr, bcnt.data(), bcnt.size(), c->s));
