@@ -292,3 +292,29 @@ def test_fp32_per_step_error_against_fp64_on_cube_column():
     for r in rows:
         print(f"{r[0]:4d} {r[1]:8d} {r[2]:8d} {r[3]:16.3e} {r[4]:16.3e} {r[5]:.3e} {r[6]:.3e} {r[7]}")
     assert rows and rows[-1][7] > 0
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_iteration_budget_edges(prec):
+    """A solve whose budget ends on the iteration that meets the tolerance is
+    converged (the test of that last residual runs after the loop); one
+    iteration fewer, or a small budget, returns not-converged (status 5)
+    with the residual above the tolerance."""
+    sc = S.two_cubes(4)
+    x = _deformed(sc, 0.004, 7)
+    g = NewtonIPC(sc, precision=prec)
+    g.set_state(x)
+    g.compute_xn_and_xTilde(0.0)
+    g.find_constraints()
+    g.compute_hessian_and_gradient()
+    n, rel = g.solve_system()
+    assert g.last_solve_converged and rel <= g.pcg_tol * 1.0001 and n > 3
+    it, r = C.c_int(), C.c_double()
+    assert g.so.fipc_solve(g.ctx, g.pcg_tol, n, C.byref(it), C.byref(r)) == 0
+    assert it.value == n and r.value <= g.pcg_tol * 1.0001
+    assert g.so.fipc_solve(g.ctx, g.pcg_tol, 3, C.byref(it), C.byref(r)) == 5
+    assert it.value == 3 and r.value > g.pcg_tol
+    if prec == 0:
+        assert g.so.fipc_solve(g.ctx, g.pcg_tol, n - 1, C.byref(it), C.byref(r)) == 5
+        assert it.value == n - 1 and r.value > g.pcg_tol
+    print(f"precision {prec}: {n} iterations to converge; budgets n, 3{', n - 1' if prec == 0 else ''} checked")
