@@ -118,10 +118,21 @@ struct fipc_ctx {
     // (oriented to its global block) to cpos[10 e + slot] of a scratch sorted
     // by block, so the gather sums contiguous runs bcptr[b] .. bcptr[b + 1];
     // the gradient is gathered through vertex -> (corner, tet) lists
-    DVec<double> tetH, tetg;
-    DVec<float> tetHf;     // the records in FP32 mode
+    DVec<double> tetH, tetg;  // FP64 mode
     DVec<int> cpos, bcptr, vcptr, vcent;
     DVec<int> gorder;      // the gather's block order: diagonal blocks first
+    // FP32 mode's tiled pass (k_elastic_tile): nchunk chunks of consecutive
+    // tets, TILE_META int4 each; per (slot, tet) and (corner, tet) the LDS
+    // index | colour << 8 of the entry it adds to; per (chunk, block) and
+    // (chunk, vertex) the position of the partial record, whose runs per
+    // block / vertex are tile_pptr / tile_vptr
+    int nchunk = 0;
+    long tile_nbpart = 0, tile_nvpart = 0;
+    DVec<int4> tile_meta;
+    DVec<unsigned short> tile_tcode, tile_vcode;
+    DVec<int> tile_ppos, tile_vpos, tile_pptr, tile_vptr;
+    DVec<float> tile_bpart;
+    DVec<double> tile_gpart;
     int precision = 0;     // 0 = FP64 Hessian, 1 = FP32 Hessian / SpMV
     // contacts
     int64_t cap[NTYPES] = {0};

@@ -179,18 +179,16 @@ __device__ __forceinline__ void tet_load(const int* tets, const double* IB, cons
 // index of the upper 3x3 block (a <= c) of a tet's 12x12 Hessian, 0..9
 __host__ __device__ constexpr int uslot(int a, int c) { return a * 4 - a * (a - 1) / 2 + (c - a); }
 
-// compute_elasticity (NEWTON_IPC.py:568-691), first pass: one thread per tet
-// writes each of its ten upper 3x3 blocks, oriented as the stored global
-// block, to tetH[cpos[10 e + slot] * 9 ..] (positions sorted by block, so the
-// gather reads contiguous runs) and its gradient to
-// tetg[(corner * nt + e) * 3 + i] (corner-major: a wave's stores are
-// contiguous)
-template <class R>
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) k_elastic(int nt, const int* __restrict__ tets, const double* __restrict__ IB,
-                                                 const double* __restrict__ vol, const double* __restrict__ la,
-                                                 const double* __restrict__ mu, const double* __restrict__ x,
-                                                 double dt2, const int* __restrict__ cpos, R* __restrict__ tetH,
-                                                 double* __restrict__ tetg)
+// compute_elasticity (NEWTON_IPC.py:568-691), first pass in FP64 mode: one
+// thread per tet writes each of its ten upper 3x3 blocks, oriented as the
+// stored global block, to tetH[cpos[10 e + slot] * 9 ..] (positions sorted by
+// block, so the gather reads contiguous runs and sums every block's tets in
+// index order) and its gradient to tetg[(corner * nt + e) * 3 + i]
+// (corner-major: a wave's stores are contiguous)
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3)))
+k_elastic(int nt, const int* __restrict__ tets, const double* __restrict__ IB, const double* __restrict__ vol,
+          const double* __restrict__ la, const double* __restrict__ mu, const double* __restrict__ x, double dt2,
+          const int* __restrict__ cpos, double* __restrict__ tetH, double* __restrict__ tetg)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nt) return;
@@ -210,24 +208,122 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3))) k
     nh_factor(F, lae, mue, f);
     // block (a, c), a <= c, belongs to the upper block (min, max) of the global
     // vertices: transposed when the global order is reversed
-    // FP32 records are computed in FP32 arithmetic from the FP64 factors
-    using T = typename std::conditional<sizeof(R) == 4, float, double>::type;
-    tet_hessian_blocks_t<T>(f, Bm, sc, [&](int a, int c, const T* blk) {
-        R* o = tetH + (size_t)cpos[(size_t)e * 10 + uslot(a, c)] * 9;
+    tet_hessian_blocks(f, Bm, sc, [&](int a, int c, const double* blk) {
+        double* o = tetH + (size_t)cpos[(size_t)e * 10 + uslot(a, c)] * 9;
         if (t[a] <= t[c]) {
 #pragma unroll
-            for (int k = 0; k < 9; k++) o[k] = (R)blk[k];
+            for (int k = 0; k < 9; k++) o[k] = blk[k];
         } else {
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) o[i * 3 + j] = (R)blk[j * 3 + i];
+                for (int j = 0; j < 3; j++) o[i * 3 + j] = blk[j * 3 + i];
         }
     });
 }
 
+// FP32 mode, tiled: one wavefront per chunk of up to TILE_T consecutive tets
+// whose distinct blocks and vertices fit the wave's LDS (the chunks are cut
+// at set-up).  The wave sums its tets' oriented FP32 blocks per distinct
+// block, and their FP64 corner gradients per distinct vertex, in LDS, then
+// writes one partial record per (chunk, block) and (chunk, vertex) to a
+// scratch sorted by block / vertex (chunks in order within a run), about a
+// third / a fifth of the per-tet records on lattice-ordered meshes.  The sums
+// are in a fixed order: slot by slot (corner by corner), and within a slot
+// the tets that hit the same entry add one after another in index order
+// ("colours" from the set-up tables), so no atomics are needed.
+constexpr int TILE_T = 64;       // tets per chunk: one per lane
+constexpr int TILE_MAXB = 256;   // distinct blocks per chunk (9 floats of LDS each)
+constexpr int TILE_MAXV = 128;   // distinct vertices per chunk (3 doubles of LDS each)
+constexpr int TILE_META = 3;     // int4 per chunk: {e0, n, nb, nv}, {pbase, vbase, -, -}, colour counts
+static_assert(TPB % 64 == 0, "whole wavefronts per workgroup");
+
+// orders the LDS accesses of the wave's lanes around a colour step (LDS
+// instructions of one wavefront execute in issue order; this keeps the
+// compiler from moving them across the step)
+__device__ __forceinline__ void wave_step()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+template <int N, class A>
+__device__ __forceinline__ void tile_add(A* lds, unsigned code, int ncol, const A (&v)[N], bool act)
+{
+    const int li = (int)(code & 0xffu), col = (int)(code >> 8);
+    for (int cc = 0; cc < ncol; cc++) {
+        if (act && col == cc) {
+#pragma unroll
+            for (int q = 0; q < N; q++) lds[N * li + q] += v[q];
+        }
+        wave_step();
+    }
+}
+
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3)))
+k_elastic_tile(int nchunk, int nt, const int4* __restrict__ meta, const unsigned short* __restrict__ tcode,
+               const unsigned short* __restrict__ vcode, const int* __restrict__ ppos, const int* __restrict__ vpos,
+               const int* __restrict__ tets, const double* __restrict__ IB, const double* __restrict__ vol,
+               const double* __restrict__ la, const double* __restrict__ mu, const double* __restrict__ x,
+               double dt2, float* __restrict__ bpart, double* __restrict__ gpart)
+{
+    __shared__ float lb[TPB / 64][TILE_MAXB * 9];
+    __shared__ double lg[TPB / 64][TILE_MAXV * 3];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)blockIdx.x * (TPB / 64) + wv);
+    if (w >= nchunk) return;
+    const int4 m0 = meta[TILE_META * w], m1 = meta[TILE_META * w + 1], m2 = meta[TILE_META * w + 2];
+    const int e0 = __builtin_amdgcn_readfirstlane(m0.x), n = __builtin_amdgcn_readfirstlane(m0.y);
+    const int nb = __builtin_amdgcn_readfirstlane(m0.z), nvx = __builtin_amdgcn_readfirstlane(m0.w);
+    const int pbase = __builtin_amdgcn_readfirstlane(m1.x), vbase = __builtin_amdgcn_readfirstlane(m1.y);
+    // colour counts: bytes 0..9 the block slots, 10..13 the corners
+    const unsigned cw[4] = {(unsigned)__builtin_amdgcn_readfirstlane(m2.x), (unsigned)__builtin_amdgcn_readfirstlane(m2.y),
+                            (unsigned)__builtin_amdgcn_readfirstlane(m2.z), (unsigned)__builtin_amdgcn_readfirstlane(m2.w)};
+    float* B = lb[wv];
+    double* G = lg[wv];
+    for (int i = lane; i < 9 * nb; i += 64) B[i] = 0.0f;
+    for (int i = lane; i < 3 * nvx; i += 64) G[i] = 0.0;
+    wave_step();
+    const bool act = lane < n;
+    const int e = act ? e0 + lane : e0;  // idle lanes recompute the first tet and add nothing
+    int t[4];
+    double F[9], Bm[4][3];
+    tet_load(tets, IB, x, e, t, F, Bm);
+    const double sc = dt2 * vol[e];
+    const double lae = la[e], mue = mu[e];
+    double P[9];
+    nh_P(F, lae, mue, P);
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        double g[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) g[i] = (P[i * 3] * Bm[a][0] + P[i * 3 + 1] * Bm[a][1] + P[i * 3 + 2] * Bm[a][2]) * sc;
+        const int k = 10 + a;
+        tile_add<3, double>(G, vcode[(size_t)a * nt + e], (int)((cw[k >> 2] >> (8 * (k & 3))) & 0xffu), g, act);
+    }
+    NHFactor f;
+    nh_factor(F, lae, mue, f);
+    tet_hessian_blocks_t<float>(f, Bm, sc, [&](int a, int c, const float* blk) {
+        const int k = uslot(a, c);
+        float o[9];
+        if (t[a] <= t[c]) {
+#pragma unroll
+            for (int q = 0; q < 9; q++) o[q] = blk[q];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) o[i * 3 + j] = blk[j * 3 + i];
+        }
+        tile_add<9, float>(B, tcode[(size_t)k * nt + e], (int)((cw[k >> 2] >> (8 * (k & 3))) & 0xffu), o, act);
+    });
+    // nine (three) consecutive lanes write a record's consecutive entries
+    for (int i = lane; i < 9 * nb; i += 64) bpart[(size_t)ppos[pbase + i / 9] * 9 + i % 9] = B[i];
+    for (int i = lane; i < 3 * nvx; i += 64) gpart[(size_t)vpos[vbase + i / 3] * 3 + i % 3] = G[i];
+}
+
 // second pass: sum each stored (upper) block's contiguous run of oriented tet
-// blocks in the scratch (tets in index order), add the lumped mass on diagonal
+// blocks in the scratch (tets in index order; in FP32 mode the chunk partials
+// of the tiled pass, chunks in order), add the lumped mass on diagonal
 // blocks (compute_inertia, NEWTON_IPC.py:558-567), store the FP64 values and,
 // when valf != nullptr, their FP32 copy for the FP32 SpMV.  Nine lanes per
 // block, one per entry, seven blocks per wavefront (lane 63 idles): the lanes
@@ -267,7 +363,8 @@ __global__ void __launch_bounds__(TPB) k_gather_blocks(int nnzb, const int* __re
 // (compute_inertia, NEWTON_IPC.py:558-567; rhs = -g, rhs[D] = 0,
 // NEWTON_IPC.py:772) and clears the contact diagonal blocks before
 // k_contact adds its rows; k_gather_tetg then subtracts the vertex's tet
-// contributions in order.  Three lanes per vertex, one per component, 21
+// contributions in order (after the tiled pass: its chunk partials in chunk
+// order).  Three lanes per vertex, one per component, 21
 // vertices per wavefront (lane 63 idles): the lanes of a vertex read each
 // 24-byte contribution together.
 constexpr int RHS_VPW = 21;                     // vertices per wavefront
@@ -296,8 +393,13 @@ __global__ void __launch_bounds__(TPB) k_gather_tetg(int nv, const int* __restri
     if (i >= nv || dmask[i]) return;
     double r = rhs[3 * i + d];
     const int t1 = vcptr[i + 1];
+    if (vcent) {
 #pragma unroll 4
-    for (int t = vcptr[i]; t < t1; t++) r -= tetg[(size_t)vcent[t] * 3 + d];
+        for (int t = vcptr[i]; t < t1; t++) r -= tetg[(size_t)vcent[t] * 3 + d];
+    } else {  // the tiled pass's partials: a contiguous run per vertex
+#pragma unroll 4
+        for (int t = vcptr[i]; t < t1; t++) r -= tetg[(size_t)t * 3 + d];
+    }
     rhs[3 * i + d] = r;
 }
 
@@ -1308,9 +1410,11 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->x); release(c->x0); release(c->xn); release(c->xt); release(c->xprev); release(c->v);
     release(c->sol); release(c->rhs); release(c->target);
     release(c->rowptr); release(c->col); release(c->diag); release(c->val); release(c->valf);
-    release(c->brow); release(c->tetH); release(c->tetHf); release(c->tetg); release(c->bcptr); release(c->gorder); release(c->cpos); release(c->vcptr);
+    release(c->brow); release(c->tetH); release(c->tetg); release(c->bcptr); release(c->gorder); release(c->cpos); release(c->vcptr);
     release(c->lptr); release(c->lent);
     release(c->vcent);
+    release(c->tile_meta); release(c->tile_tcode); release(c->tile_vcode); release(c->tile_ppos); release(c->tile_vpos);
+    release(c->tile_pptr); release(c->tile_vptr); release(c->tile_bpart); release(c->tile_gpart);
     for (int t = 0; t < NTYPES; t++) { release(c->rows[t]); release(c->cH[t]); release(c->cHf[t]); release(c->c64[t]); }
     release(c->cv_keys); release(c->cv_vals); release(c->cv_keys2); release(c->cv_slot); release(c->cv_head);
     release(c->cv_vert); release(c->cv_ptr); release(c->cv_count); release(c->cgrad);
@@ -1338,6 +1442,82 @@ extern "C" int fipc_destroy(fipc_ctx* c)
 // symmetric BSR pattern of the elastic + inertia Hessian: one upper block per
 // vertex pair sharing a tet (and every diagonal block), the lower lists of the
 // SpMV, and the block-sorted positions of the per-tet contributions
+// chunks of the tiled FP32 pass: greedy over the tets in index order, a
+// chunk closes at TILE_T tets or when the next tet's new blocks or vertices
+// would overflow the wave's LDS (TILE_MAXB, TILE_MAXV); a single tet always
+// fits.  Colour of (slot k, tet e): how many earlier tets of the chunk add to
+// the same block in slot k (for vertices: in corner a).
+static int build_tiles(fipc_ctx* c, int nv, int nt, int nnzb, const int* tets, const std::vector<int>& cblock)
+{
+    std::vector<int4> meta;
+    std::vector<unsigned short> tcode((size_t)std::max(nt, 1) * 10), vcode((size_t)std::max(nt, 1) * 4);
+    std::vector<int> cblk, cvert;  // per (chunk, local index): the block / vertex
+    std::vector<int> bstamp(std::max(nnzb, 1), -1), blid(std::max(nnzb, 1)), vstamp(nv, -1), vlid(nv);
+    std::vector<int> ccnt(14 * (size_t)TILE_MAXB);
+    int e = 0, ch = 0;
+    while (e < nt) {
+        const int e0 = e;
+        int nb = 0, nvx = 0;
+        unsigned char ncol[16] = {0};
+        std::fill(ccnt.begin(), ccnt.end(), 0);
+        const int pbase = (int)cblk.size(), vbase = (int)cvert.size();
+        while (e < nt && e - e0 < TILE_T) {
+            int newb = 0, newv = 0;
+            for (int k = 0; k < 10; k++) newb += bstamp[cblock[(size_t)e * 10 + k]] != ch;
+            for (int a = 0; a < 4; a++) newv += vstamp[tets[4 * e + a]] != ch;
+            if (nb + newb > TILE_MAXB || nvx + newv > TILE_MAXV) break;
+            for (int k = 0; k < 10; k++) {
+                int b = cblock[(size_t)e * 10 + k];
+                if (bstamp[b] != ch) { bstamp[b] = ch; blid[b] = nb++; cblk.push_back(b); }
+                int col = ccnt[(size_t)k * TILE_MAXB + blid[b]]++;
+                tcode[(size_t)k * nt + e] = (unsigned short)(blid[b] | col << 8);
+                ncol[k] = (unsigned char)std::max<int>(ncol[k], col + 1);
+            }
+            for (int a = 0; a < 4; a++) {
+                int v = tets[4 * e + a];
+                if (vstamp[v] != ch) { vstamp[v] = ch; vlid[v] = nvx++; cvert.push_back(v); }
+                int col = ccnt[(size_t)(10 + a) * TILE_MAXB + vlid[v]]++;
+                vcode[(size_t)a * nt + e] = (unsigned short)(vlid[v] | col << 8);
+                ncol[10 + a] = (unsigned char)std::max<int>(ncol[10 + a], col + 1);
+            }
+            e++;
+        }
+        int cw[4];
+        memcpy(cw, ncol, 16);
+        meta.push_back(make_int4(e0, e - e0, nb, nvx));
+        meta.push_back(make_int4(pbase, vbase, 0, 0));
+        meta.push_back(make_int4(cw[0], cw[1], cw[2], cw[3]));
+        ch++;
+    }
+    // partial positions: sorted by block (vertex), chunks in order within a run
+    auto place = [](const std::vector<int>& ids, int n, std::vector<int>& ptr, std::vector<int>& pos) {
+        ptr.assign(n + 1, 0);
+        for (int id : ids) ptr[id + 1]++;
+        for (int i = 0; i < n; i++) ptr[i + 1] += ptr[i];
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        pos.resize(std::max<size_t>(ids.size(), 1));
+        for (size_t k = 0; k < ids.size(); k++) pos[k] = fill[ids[k]]++;
+    };
+    std::vector<int> pptr, ppos, vptr, vpos;
+    place(cblk, nnzb, pptr, ppos);
+    place(cvert, nv, vptr, vpos);
+    c->nchunk = ch;
+    if (meta.empty()) meta.push_back(make_int4(0, 0, 0, 0));
+    STCHK(upload(c->tile_meta, meta.data(), meta.size(), c->s));
+    STCHK(upload(c->tile_tcode, tcode.data(), tcode.size(), c->s));
+    STCHK(upload(c->tile_vcode, vcode.data(), vcode.size(), c->s));
+    STCHK(upload(c->tile_ppos, ppos.data(), ppos.size(), c->s));
+    STCHK(upload(c->tile_vpos, vpos.data(), vpos.size(), c->s));
+    STCHK(upload(c->tile_pptr, pptr.data(), pptr.size(), c->s));
+    STCHK(upload(c->tile_vptr, vptr.data(), vptr.size(), c->s));
+    // the partial scratch itself is allocated by the first FP32 assembly
+    c->tile_nbpart = (long)cblk.size();
+    c->tile_nvpart = (long)cvert.size();
+    // upload() is asynchronous from the host vectors: wait before they go
+    HIPCHK(hipStreamSynchronize(c->s));
+    return FIPC_OK;
+}
+
 static int build_pattern(fipc_ctx* c, int nv, int nt, const int* tets)
 {
     // upper pattern: row i holds i and its neighbours j > i, sorted
@@ -1413,6 +1593,7 @@ static int build_pattern(fipc_ctx* c, int nv, int nt, const int* tets)
         for (int i = 0; i < nv; i++)
             for (int b = rowptr[i] + 1; b < rowptr[i + 1]; b++) gorder[k++] = b;
     }
+    STCHK(build_tiles(c, nv, nt, nnzb, tets, cblock));
     STCHK(upload(c->gorder, gorder.data(), gorder.size(), c->s));
     STCHK(upload(c->bcptr, bcnt.data(), bcnt.size(), c->s));
     STCHK(upload(c->cpos, cpos.data(), cpos.size(), c->s));
@@ -1421,8 +1602,6 @@ static int build_pattern(fipc_ctx* c, int nv, int nt, const int* tets)
     STCHK(upload(c->lent, lent.data(), lent.size(), c->s));
     STCHK(upload(c->vcptr, vcnt.data(), vcnt.size(), c->s));
     STCHK(upload(c->vcent, vcent.data(), vcent.size(), c->s));
-    STCHK(ensure(c->tetH, (size_t)std::max(nt, 1) * 90));
-    STCHK(ensure(c->tetg, (size_t)std::max(nt, 1) * 12));
     c->nnzb = nnzb;
     c->nlower = nlower;
     c->nnzb_full = nnzb + nlower;
@@ -1736,10 +1915,12 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     if (precision == 1) {
         STCHK(ensure(c->valf, (size_t)c->nnzb * 9));
         STCHK(ensure(c->Minvf, (size_t)c->nv * 9));
-        STCHK(ensure(c->tetHf, (size_t)std::max(c->nt, 1) * 90));
+        STCHK(ensure(c->tile_bpart, (size_t)std::max(c->tile_nbpart, 1L) * 9));
+        STCHK(ensure(c->tile_gpart, (size_t)std::max(c->tile_nvpart, 1L) * 3));
     } else {
         STCHK(ensure(c->val, (size_t)c->nnzb * 9));
         STCHK(ensure(c->tetH, (size_t)std::max(c->nt, 1) * 90));
+        STCHK(ensure(c->tetg, (size_t)std::max(c->nt, 1) * 12));
     }
     // the contact rows (few, latency-bound) run on the second stream while
     // the tet pass and the block gather fill the GPU: the second stream
@@ -1753,31 +1934,40 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     STCHK(launch_contacts(c, 1, c->s2));
     STCHK(fipc_tm_end(c, TM_CONTACT, c->s2));
     HIPCHK(hipEventRecord(c->ev_contact, c->s2));
-    // FP64: the per-tet records and the block values are FP64.  FP32: the
-    // records are rounded to FP32 and each block sums them in FP64 and
-    // stores the FP32 value only (the refinement's residual products read
-    // the same FP32 values in FP64 arithmetic)
+    // FP64: per-tet FP64 records, each block sums its tets in index order.
+    // FP32: the tiled pass sums FP32 blocks per chunk in LDS, each block sums
+    // its chunk partials in FP64 and stores the FP32 value only (the
+    // refinement's residual products read the same FP32 values in FP64
+    // arithmetic)
     STCHK(fipc_tm_begin(c, TM_ELASTIC));
     if (c->nt) {
         if (precision == 1)
-            k_elastic<float><<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p,
-                                                             c->x.p, c->dt * c->dt, c->cpos.p, c->tetHf.p, c->tetg.p);
+            k_elastic_tile<<<(c->nchunk + TPB / 64 - 1) / (TPB / 64), TPB, 0, c->s>>>(
+                c->nchunk, c->nt, c->tile_meta.p, c->tile_tcode.p, c->tile_vcode.p, c->tile_ppos.p, c->tile_vpos.p,
+                c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p, c->x.p, c->dt * c->dt, c->tile_bpart.p,
+                c->tile_gpart.p);
         else
-            k_elastic<double><<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p,
+            k_elastic<<<nblk(c->nt), TPB, 0, c->s>>>(c->nt, c->tets.p, c->IB.p, c->vol.p, c->la.p, c->mu.p,
                                                               c->x.p, c->dt * c->dt, c->cpos.p, c->tetH.p, c->tetg.p);
     }
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_ELASTIC));
     STCHK(fipc_tm_begin(c, TM_GATHER));
     if (precision == 1)
-        k_gather_blocks<float><<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->gorder.p, c->bcptr.p, c->tetHf.p, c->brow.p,
-                                                                        c->col.p, c->m.p, nullptr, c->valf.p);
+        k_gather_blocks<float><<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->gorder.p, c->tile_pptr.p,
+                                                                        c->tile_bpart.p, c->brow.p, c->col.p, c->m.p,
+                                                                        nullptr, c->valf.p);
     else
         k_gather_blocks<double><<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->gorder.p, c->bcptr.p, c->tetH.p, c->brow.p,
                                                                          c->col.p, c->m.p, c->val.p, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamWaitEvent(c->s, c->ev_contact, 0));
-    k_gather_tetg<<<rhs_grid(c->nv), TPB, 0, c->s>>>(c->nv, c->vcptr.p, c->vcent.p, c->tetg.p, c->dmask.p, c->rhs.p);
+    if (precision == 1)
+        k_gather_tetg<<<rhs_grid(c->nv), TPB, 0, c->s>>>(c->nv, c->tile_vptr.p, nullptr, c->tile_gpart.p, c->dmask.p,
+                                                         c->rhs.p);
+    else
+        k_gather_tetg<<<rhs_grid(c->nv), TPB, 0, c->s>>>(c->nv, c->vcptr.p, c->vcent.p, c->tetg.p, c->dmask.p,
+                                                         c->rhs.p);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, TM_GATHER));
     if (precision == 1)

@@ -190,11 +190,14 @@ int fipc_search_counts(fipc_ctx* c, int* cand);
 int fipc_get_constraints(fipc_ctx* c, int type, int* rows_out);
 /* output_current_minimal_distance (NEWTON_IPC.py:815-839) */
 int fipc_min_distance(fipc_ctx* c, double* d2min);
-/* compute_hessian_and_gradient (NEWTON_IPC.py:736-747): precision 0 = FP64,
- * 1 = FP32 Hessian storage (elastic blocks summed in FP64 and stored in FP32;
- * contact rows in FP32 too except those closer than 1e-2 dHat).  The contact
- * rows' gradient and diagonal blocks are summed per vertex in a fixed order
- * (no atomics), so the result is the same bits on every run. */
+/* compute_hessian_and_gradient (NEWTON_IPC.py:736-747): precision 0 = FP64
+ * (every block and gradient entry sums its tets in index order),
+ * 1 = FP32 Hessian storage (FP32 tet blocks summed per chunk of up to 64
+ * consecutive tets in FP32, the chunk partials in FP64, stored in FP32; the
+ * gradient stays FP64, summed per chunk, then over chunks; contact rows in
+ * FP32 too except those closer than 1e-2 dHat).  The contact rows' gradient
+ * and diagonal blocks are summed per vertex in a fixed order; no sum uses
+ * atomics, so the result is the same bits on every run. */
 int fipc_assemble(fipc_ctx* c, int precision);
 /* solve_system (NEWTON_IPC.py:750-777) with block-Jacobi PCG instead of
  * CHOLMOD: relative residual tolerance tol, at most maxit iterations */

@@ -87,8 +87,8 @@ def test_fp32_solve_meets_fp64_residual(tol):
 
 
 def test_fp32_hessian_blocks_are_rounded_fp64():
-    """FP32 mode stores each elastic block as the FP32 rounding of the sum,
-    accumulated in FP64, of the FP32-rounded per-tet contributions: every
+    """FP32 mode stores each elastic block as the FP32 rounding of an FP64 sum
+    of chunk partials, each an FP32 sum of FP32 per-tet contributions: every
     entry within a few FP32 units of the FP64 block (relative to the
     block's largest entry), and the two modes share the pattern."""
     sc = S.two_cubes(4)
@@ -113,6 +113,51 @@ def test_fp32_hessian_blocks_are_rounded_fp64():
     print(f"FP32 block entries: max error {err.max():.3e} of the block's largest entry")
     assert err.max() <= 2.0 ** -18
     assert np.array_equal(v1, v1.astype(np.float32).astype(np.float64))
+
+
+def _assembled(sc, x, prec):
+    g = NewtonIPC(sc, precision=prec)
+    g.set_state(x)
+    g.compute_xn_and_xTilde(0.0)
+    g.find_constraints()
+    out = []
+    for _ in range(2):
+        g.compute_hessian_and_gradient()
+        n = C.c_int()
+        _lib.call("fipc_get_bsr", g.ctx, C.byref(n), None, None, None)
+        v = np.zeros(n.value * 9)
+        _lib.call("fipc_get_bsr", g.ctx, C.byref(n), None, None, _lib.dptr(v))
+        out.append((g.vector(2).ravel().copy(), v.reshape(-1, 9)))
+    return out
+
+
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_fp32_tiled_assembly_matches_fp64(shuffle):
+    """FP32 mode's tiled tet pass (a wavefront sums up to 64 consecutive tets'
+    blocks and corner gradients in LDS and writes one partial per distinct
+    block and vertex) against FP64 mode's per-tet records, on a deformed
+    10x16x10 compressed slab: the gradient (FP64 in both modes; only the
+    summation order differs) within 1e-12 of its largest entry, the blocks
+    within 2^-18 of each block's largest entry, and a second assembly gives
+    the same bits.  With the tets shuffled, consecutive tets share few
+    vertices and the chunks close on the LDS limit of 256 distinct blocks
+    instead of at 64 tets."""
+    import dataclasses
+    sc = S.compressed_slab(10, 16, 10)
+    if shuffle:
+        perm = np.random.default_rng(7).permutation(len(sc.tets))
+        sc = dataclasses.replace(sc, tets=np.ascontiguousarray(sc.tets[perm]), density=sc.density[perm],
+                                 youngs=sc.youngs[perm], poisson=sc.poisson[perm])
+    x = _deformed(sc, 0.02, 5)
+    (r64, v64), _ = _assembled(sc, x, 0)
+    (r32, v32), (r32b, v32b) = _assembled(sc, x, 1)
+    gerr = np.abs(r32 - r64).max() / np.abs(r64).max()
+    scale = np.abs(v64).max(axis=1, keepdims=True)
+    berr = (np.abs(v32 - v64) / scale).max()
+    print(f"shuffle={shuffle}: gradient error {gerr:.3e} of its largest entry, block error {berr:.3e}")
+    assert gerr <= 1e-12
+    assert berr <= 2.0 ** -18
+    assert np.array_equal(r32, r32b) and np.array_equal(v32, v32b)
 
 
 @pytest.mark.parametrize("scene_fn,steps,amp", [(S.tiny_pair, 4, 0.001), (lambda: S.two_cubes(4), 12, 0.004)])
