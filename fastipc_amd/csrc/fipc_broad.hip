@@ -89,26 +89,112 @@ struct BoxFn {
 #define GRID_MAXB 2048
 static inline int stride_grid(long n) { return std::max(1, std::min(nblk(n), GRID_MAXB)); }
 
-__global__ void k_grid_count(int n, BoxFn box, double inv, int* cnt, int* flag)
+// ---- two-level offsets ----------------------------------------------------
+// The entries of n boxes (one per overlapped cell) are numbered without a
+// device-wide scan: k_count_cells stores each box's offset within its block
+// of TPB boxes (loc) and the block's total (part); a consumer workgroup
+// prefix-sums the ≤ OFF_PMAX block totals in LDS, and the offset of box i is
+// P[i / TPB] + loc[i].  Saves a scan launch pair per grid build and query.
+constexpr int OFF_PMAX = 8192;  // block totals per table: up to 2M boxes (32 KB of LDS)
+
+// exclusive prefix of v over the workgroup (TPB threads), *total = the sum
+__device__ __forceinline__ long long block_exscan(long long v, long long* total)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
-    if (i == n) { cnt[n] = 0; return; }
-    v3 lo, hi;
-    box(i, lo, hi);
-    long long c = ncells(cells_of(lo, hi, inv));
-    if (c > (1 << 24) || c < 0) { atomicOr(flag, GF_HUGE); c = 0; }
-    cnt[i] = (int)c;
+    __shared__ long long wsum[TPB / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    long long x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        long long y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    long long before = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < TPB / 64; k++) {
+        if (k < w) before += wsum[k];
+        all += wsum[k];
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
 }
-// Grid-stride over the entries (ne = off[n], scanned on the device, so the
-// host never waits for it).  A count above the capacity flags a retry and
-// records the count needed.  The owning item of an entry is found by binary
-// search in the scanned counts, so an item covering thousands of cells (a
-// long plate edge) does not serialise a wave.
-__global__ void k_grid_fill(int n, int cap, BoxFn box, double inv, unsigned M, const int* __restrict__ off,
-                            unsigned* ekey, int* eitem, unsigned long long* ecell, int* bcount, int* flag)
+// cells of every box: loc[i] = offset of box i within its block, part[b] =
+// the block's total; a box over more than 2^24 cells, or a block over 2^30,
+// flags GF_HUGE (the operation fails with FIPC_ERR_CAPACITY)
+__global__ void __launch_bounds__(TPB) k_count_cells(int n, BoxFn box, double inv, int* loc, int* part, int* flag)
 {
-    int ne = off[n];
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    long long c = 0;
+    if (i < n) {
+        v3 lo, hi;
+        box(i, lo, hi);
+        c = ncells(cells_of(lo, hi, inv));
+        if (c > (1 << 24) || c < 0) { atomicOr(flag, GF_HUGE); c = 0; }
+    }
+    long long tot;
+    const long long pre = block_exscan(c, &tot);
+    if (tot >= (1LL << 30)) {
+        if (threadIdx.x == 0) atomicOr(flag, GF_HUGE);
+        tot = 0;
+    }
+    if (i < n) loc[i] = (int)pre;
+    if (threadIdx.x == 0) part[blockIdx.x] = (int)tot;
+}
+// sP[0..np] = exclusive prefix of part[0..np) in LDS (every thread of the
+// workgroup calls it); returns the total, or -1 past 2^31 - 1
+__device__ int lds_prefix(const int* __restrict__ part, int np, int* sP)
+{
+    for (int k = threadIdx.x; k < np; k += TPB) sP[k] = part[k];
+    __syncthreads();
+    const int per = (np + TPB - 1) / TPB, k0 = min(np, (int)threadIdx.x * per), k1 = min(np, k0 + per);
+    long long sum = 0;
+    for (int k = k0; k < k1; k++) sum += sP[k];
+    long long tot;
+    long long pre = block_exscan(sum, &tot);
+    if (tot > 0x7fffffffLL) tot = -1, pre = 0;
+    for (int k = k0; k < k1; k++) {
+        const int v = sP[k];
+        sP[k] = (int)pre;
+        pre += v;
+    }
+    __syncthreads();
+    return (int)tot;
+}
+// the box whose entries hold entry e (0 <= e < total): the last box with
+// offset <= e (boxes with no cells share the next box's offset); *first = its
+// first entry
+__device__ __forceinline__ int find_box(int e, const int* sP, int np, const int* __restrict__ loc, int n, int* first)
+{
+    int lo = 0, hi = np;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (sP[mid] <= e) lo = mid; else hi = mid;
+    }
+    const int r = e - sP[lo];
+    int l = lo * TPB, h = min(n, l + TPB);
+    while (h - l > 1) {
+        const int mid = (l + h) >> 1;
+        if (loc[mid] <= r) l = mid; else h = mid;
+    }
+    *first = sP[lo] + loc[l];
+    return l;
+}
+
+// Grid-stride over the entries (their number is the sum of the block totals,
+// so the host never waits for it).  A count above the capacity flags a retry
+// and records the count needed; workgroup 0 stores the count to part[np]
+// for the scatter.  The owning item of an entry is found by binary search
+// in the offsets, so an item covering thousands of cells (a long plate edge)
+// does not serialise a wave.  Dynamic LDS: (np + 1) ints.
+__global__ void __launch_bounds__(TPB) k_grid_fill(int n, int np, int cap, BoxFn box, double inv, unsigned M,
+                                                   const int* __restrict__ loc, int* part, unsigned* ekey,
+                                                   int* eitem, unsigned long long* ecell, int* bcount, int* flag)
+{
+    extern __shared__ int sP[];
+    const int ne = lds_prefix(part, np, sP);
+    if (blockIdx.x == 0 && threadIdx.x == 0) part[np] = ne;
     if (ne > cap || ne < 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             atomicOr(flag, ne < 0 ? GF_HUGE : GF_RETRY);
@@ -117,17 +203,13 @@ __global__ void k_grid_fill(int n, int cap, BoxFn box, double inv, unsigned M, c
         return;
     }
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
-        int lo = 0, hi = n;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (off[mid] <= e) lo = mid; else hi = mid;
-        }
-        int i = lo;
+        int first;
+        const int i = find_box(e, sP, np, loc, n, &first);
         v3 blo, bhi;
         box(i, blo, bhi);
         CellRange r = cells_of(blo, bhi, inv);
         int ny = r.hi[1] - r.lo[1] + 1, nz = r.hi[2] - r.lo[2] + 1;
-        int l = e - off[i];
+        int l = e - first;
         int a = r.lo[0] + l / (ny * nz), b = r.lo[1] + (l / nz) % ny, c = r.lo[2] + l % nz;
         unsigned flags = (a == r.lo[0] ? 1u : 0u) | (b == r.lo[1] ? 2u : 0u) | (c == r.lo[2] ? 4u : 0u);
         unsigned k = bucket_of(a, b, c, M);
@@ -144,7 +226,7 @@ __global__ void k_grid_scatter(const int* __restrict__ nep, int cap, const unsig
                                const unsigned long long* ecell, int* bstart, int* items, unsigned long long* cells)
 {
     int ne = *nep;
-    if (ne > cap) return;
+    if (ne > cap || ne < 0) return;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
         unsigned k = ekey[e];
         int pos = atomicAdd(&bstart[k], 1);
@@ -152,43 +234,34 @@ __global__ void k_grid_scatter(const int* __restrict__ nep, int cap, const unsig
         cells[pos] = ecell[e];
     }
 }
-// number of cells of every query box
-__global__ void k_query_cells(int n, BoxFn box, double inv, int* cnt, int* flag)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
-    if (i == n) { cnt[n] = 0; return; }
-    v3 lo, hi;
-    box(i, lo, hi);
-    long long c = ncells(cells_of(lo, hi, inv));
-    if (c > (1 << 24) || c < 0) { atomicOr(flag, GF_HUGE); c = 0; }
-    cnt[i] = (int)c;
-}
 
 // Generic query, one thread per (query, cell) pair, grid-stride: visit(q,
 // item) is called once per (query, item) pair whose cell ranges overlap,
 // in the first cell common to both.  Does nothing when the grid build
 // flagged a retry (the caller then redoes the whole operation).
 template <class Visit>
-__global__ void __launch_bounds__(TPB) k_grid_query(int nq, const int* __restrict__ qoff, BoxFn qbox, double inv,
+__global__ void __launch_bounds__(TPB) k_grid_query(int nq, int np, const int* __restrict__ qloc,
+                                                    const int* __restrict__ qpart, BoxFn qbox, double inv,
                                                     unsigned M, const int* __restrict__ bstart,
                                                     const int* __restrict__ items,
                                                     const unsigned long long* __restrict__ cells, int* flag,
                                                     Visit visit)
 {
-    if (flag[0] & (GF_RETRY | GF_HUGE)) return;
-    int total = qoff[nq];
+    extern __shared__ int sP[];
+    // the whole workgroup leaves or stays (lds_prefix synchronises it)
+    __shared__ int stop;
+    if (threadIdx.x == 0) stop = flag[0] & (GF_RETRY | GF_HUGE);
+    __syncthreads();
+    if (stop) return;
+    const int total = lds_prefix(qpart, np, sP);
     if (total < 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(flag, GF_HUGE);
         return;
     }
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-        int lo = 0, hi = nq;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (qoff[mid] <= t) lo = mid; else hi = mid;
-        }
-        int q = lo, l = t - qoff[q];
+        int first;
+        const int q = find_box(t, sP, np, qloc, nq, &first);
+        const int l = t - first;
         v3 blo, bhi;
         qbox(q, blo, bhi);
         CellRange r = cells_of(blo, bhi, inv);
@@ -239,6 +312,8 @@ static int grid_build(fipc_ctx* c, int slot, int n, const BoxFn& box, hipStream_
             return FIPC_OK;
     }
     g.reusable = false;
+    const int np = nblk(n);
+    if (np > OFF_PMAX) return FIPC_ERR_CAPACITY;
     STCHK(fipc_tm_begin(c, TM_GRID, s));
     int cap = g.cap;
     // buckets: a power of two at least the entry capacity
@@ -246,17 +321,17 @@ static int grid_build(fipc_ctx* c, int slot, int n, const BoxFn& box, hipStream_
     while (M < (unsigned)cap && M < (1u << 30)) M <<= 1;
     g.M = M;
     STCHK(ensure(g.cnt, (size_t)n + 2));
+    STCHK(ensure(g.part, (size_t)np + 2));
     STCHK(ensure(g.ekey, cap)); STCHK(ensure(g.eitem, cap)); STCHK(ensure(g.items, cap));
     STCHK(ensure(g.ecell, cap)); STCHK(ensure(g.cells, cap));
     STCHK(ensure(g.bstart, (size_t)M + 1));
-    k_grid_count<<<nblk(n + 1), TPB, 0, s>>>(n, box, c->inv_dx, g.cnt.p, c->flag.p);
+    if (np) k_count_cells<<<np, TPB, 0, s>>>(n, box, c->inv_dx, g.cnt.p, g.part.p, c->flag.p);
     HIPCHK(hipGetLastError());
-    STCHK(scan_inplace(c, g.cnt.p, n + 1, s));
-    const int* nep = g.cnt.p + n;
+    const int* nep = g.part.p + np;  // the entry count, stored by k_grid_fill
     int nb = stride_grid(cap);
     HIPCHK(hipMemsetAsync(g.bstart.p, 0, ((size_t)M + 1) * 4, s));
-    k_grid_fill<<<nb, TPB, 0, s>>>(n, cap, box, c->inv_dx, M, g.cnt.p, g.ekey.p, g.eitem.p, g.ecell.p, g.bstart.p,
-                                   c->flag.p);
+    k_grid_fill<<<nb, TPB, (np + 1) * sizeof(int), s>>>(n, np, cap, box, c->inv_dx, M, g.cnt.p, g.part.p, g.ekey.p,
+                                                         g.eitem.p, g.ecell.p, g.bstart.p, c->flag.p);
     HIPCHK(hipGetLastError());
     STCHK(scan_inplace(c, g.bstart.p, M + 1, s));
     k_grid_scatter<<<nb, TPB, 0, s>>>(nep, cap, g.ekey.p, g.eitem.p, g.ecell.p, g.bstart.p, g.items.p, g.cells.p);
@@ -283,12 +358,15 @@ static int run_query(fipc_ctx* c, int slot, int nq, const BoxFn& qbox, Visit vis
     if (nq == 0) return FIPC_OK;
     const hipStream_t s = st ? st : c->s;
     Grid& g = c->grid[slot];
+    const int np = nblk(nq);
+    if (np > OFF_PMAX) return FIPC_ERR_CAPACITY;
     STCHK(ensure(g.qoff, (size_t)nq + 2));
+    STCHK(ensure(g.qpart, (size_t)np + 2));
     STCHK(fipc_tm_begin(c, timer, s));
-    k_query_cells<<<nblk(nq + 1), TPB, 0, s>>>(nq, qbox, c->inv_dx, g.qoff.p, c->flag.p);
+    k_count_cells<<<np, TPB, 0, s>>>(nq, qbox, c->inv_dx, g.qoff.p, g.qpart.p, c->flag.p);
     HIPCHK(hipGetLastError());
-    STCHK(scan_inplace(c, g.qoff.p, nq + 1, s));
-    k_grid_query<<<GRID_MAXB, TPB, 0, s>>>(nq, g.qoff.p, qbox, c->inv_dx, g.M, g.bstart.p, g.items.p, g.cells.p,
+    k_grid_query<<<GRID_MAXB, TPB, (np + 1) * sizeof(int), s>>>(nq, np, g.qoff.p, g.qpart.p, qbox, c->inv_dx, g.M,
+                                                                 g.bstart.p, g.items.p, g.cells.p,
                                              c->flag.p, visit);
     HIPCHK(hipGetLastError());
     STCHK(fipc_tm_end(c, timer, s));

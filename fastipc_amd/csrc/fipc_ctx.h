@@ -64,13 +64,14 @@ static inline void release(DVec<T>& v)
 struct Grid {
     unsigned M = 0;        // bucket count (power of two, >= cap)
     int cap = 0;           // entry capacity; grows when a build overflows
-    DVec<int> cnt;         // per item cell count, scanned in place (n+1); cnt[n] = entries
+    DVec<int> cnt;         // per item: first entry within its block of items (two-level offsets)
+    DVec<int> part;        // per block of items: its entries; part[blocks] = all entries
     DVec<unsigned> ekey;   // bucket of each entry
     DVec<int> eitem;       // item | lowest-cell flags << 28
     DVec<unsigned long long> ecell, cells;  // exact cell key per entry / ordered by bucket
     DVec<int> bstart;      // M+1 bucket offsets (bucket ends after the scatter)
     DVec<int> items;       // entries ordered by bucket
-    DVec<int> qoff;        // query cell offsets (nq+1)
+    DVec<int> qoff, qpart; // query cell offsets within blocks of queries, per-block totals
     // what the grid holds, for reuse by a later operation: the x version,
     // primitive kind, count and inflation of a grid of boxes at x; for a grid
     // of swept boxes (x to x + sol), the x and sol versions it sweeps between

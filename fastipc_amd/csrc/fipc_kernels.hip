@@ -1428,7 +1428,7 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->epart); release(c->amin); release(c->flag);
     for (Grid& g : c->grid) {
         release(g.cnt); release(g.ekey); release(g.eitem); release(g.bstart);
-        release(g.items); release(g.ecell); release(g.cells); release(g.qoff);
+        release(g.items); release(g.ecell); release(g.cells); release(g.qoff); release(g.part); release(g.qpart);
     }
     release(c->ccd_pairs[0]); release(c->ccd_pairs[1]);
     release(c->dprev); release(c->wsX); release(c->ckey); release(c->free_ids); release(c->cap_ids);
