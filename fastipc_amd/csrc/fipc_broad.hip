@@ -123,9 +123,12 @@ __device__ __forceinline__ long long block_exscan(long long v, long long* total)
 // cells of every box: loc[i] = offset of box i within its block, part[b] =
 // the block's total; a box over more than 2^24 cells, or a block over 2^30,
 // flags GF_HUGE (the operation fails with FIPC_ERR_CAPACITY)
-__global__ void __launch_bounds__(TPB) k_count_cells(int n, BoxFn box, double inv, int* loc, int* part, int* flag)
+__global__ void __launch_bounds__(TPB) k_count_cells(int n, BoxFn box, double inv, int* loc, int* part, int* flag,
+                                                     int* zero = nullptr, long nzero = 0)
 {
     const int i = blockIdx.x * TPB + threadIdx.x;
+    // a grid build clears its bucket counts here (the fill adds to them)
+    for (long k = i; k < nzero; k += (long)gridDim.x * TPB) zero[k] = 0;
     long long c = 0;
     if (i < n) {
         v3 lo, hi;
@@ -325,11 +328,13 @@ static int grid_build(fipc_ctx* c, int slot, int n, const BoxFn& box, hipStream_
     STCHK(ensure(g.ekey, cap)); STCHK(ensure(g.eitem, cap)); STCHK(ensure(g.items, cap));
     STCHK(ensure(g.ecell, cap)); STCHK(ensure(g.cells, cap));
     STCHK(ensure(g.bstart, (size_t)M + 1));
-    if (np) k_count_cells<<<np, TPB, 0, s>>>(n, box, c->inv_dx, g.cnt.p, g.part.p, c->flag.p);
+    if (np)
+        k_count_cells<<<np, TPB, 0, s>>>(n, box, c->inv_dx, g.cnt.p, g.part.p, c->flag.p, g.bstart.p, (long)M + 1);
+    else
+        HIPCHK(hipMemsetAsync(g.bstart.p, 0, ((size_t)M + 1) * 4, s));
     HIPCHK(hipGetLastError());
     const int* nep = g.part.p + np;  // the entry count, stored by k_grid_fill
     int nb = stride_grid(cap);
-    HIPCHK(hipMemsetAsync(g.bstart.p, 0, ((size_t)M + 1) * 4, s));
     k_grid_fill<<<nb, TPB, (np + 1) * sizeof(int), s>>>(n, np, cap, box, c->inv_dx, M, g.cnt.p, g.part.p, g.ekey.p,
                                                          g.eitem.p, g.ecell.p, g.bstart.p, c->flag.p);
     HIPCHK(hipGetLastError());
@@ -373,10 +378,20 @@ static int run_query(fipc_ctx* c, int slot, int nq, const BoxFn& qbox, Visit vis
     return FIPC_OK;
 }
 
-// start of a broad-phase operation: clear the status word and needed count
-static int bp_begin(fipc_ctx* c)
+// start of a broad-phase operation, one launch: clears the status word and
+// needed count (flag[0..2]) and ncnt counters, and sets namin minima to 1.0
+__global__ void k_bp_begin(int* flag, int* cnt, int ncnt, unsigned long long* amin, int namin)
 {
-    HIPCHK(hipMemsetAsync(c->flag.p, 0, 12, c->s));
+    const int i = threadIdx.x;
+    if (i < 3) flag[i] = 0;
+    if (i < ncnt) cnt[i] = 0;
+    if (i < namin) amin[i] = 0x3ff0000000000000ull;  // the bits of 1.0
+}
+static int bp_begin(fipc_ctx* c, int* cnt = nullptr, int ncnt = 0, unsigned long long* amin = nullptr,
+                    int namin = 0)
+{
+    k_bp_begin<<<1, 64, 0, c->s>>>(c->flag.p, cnt, ncnt, amin, namin);
+    HIPCHK(hipGetLastError());
     return FIPC_OK;
 }
 // after the operation's synchronisation: error, retry (capacity grown) or done
@@ -570,8 +585,7 @@ static int bits_for(int n)
 // the single synchronisation
 static int find_once(fipc_ctx* c, int* counts, int* hflag)
 {
-    STCHK(bp_begin(c));
-    HIPCHK(hipMemsetAsync(c->counters.p, 0, 12 * 4, c->s));
+    STCHK(bp_begin(c, c->counters.p, 12));
     RowSink sink;
     sink.cnt = c->counters.p;
     // one same-address atomic per wavefront and candidate test would
@@ -794,13 +808,8 @@ extern "C" int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* al
         if (attempt == 6) return FIPC_ERR_CAPACITY;
         int cap = c->ccd_cap;
         STCHK(ensure(c->ccd_pairs[0], cap)); STCHK(ensure(c->ccd_pairs[1], cap));
-        STCHK(bp_begin(c));
         int* pc = c->counters.p + 8;
-        HIPCHK(hipMemsetAsync(pc, 0, 8, c->s));
-        double one = 1.0;
-        unsigned long long init[3];
-        for (int i = 0; i < 3; i++) memcpy(&init[i], &one, 8);
-        HIPCHK(hipMemcpyAsync(c->amin.p, init, 24, hipMemcpyHostToDevice, c->s));
+        STCHK(bp_begin(c, pc, 2, c->amin.p, 3));
         // the edge-edge half (the longer one) runs on the second stream while
         // the point-triangle half and the inversion bound run on the first:
         // separate grids, pair lists, counters and minima; the first waits

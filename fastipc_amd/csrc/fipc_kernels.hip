@@ -333,8 +333,10 @@ k_elastic_tile(int nchunk, int nt, const int4* __restrict__ meta, const unsigned
 constexpr int GATHER_BPW = 7;                       // blocks per wavefront
 constexpr int GATHER_BPG = GATHER_BPW * (TPB / 64); // blocks per workgroup
 static inline int gather_grid(long nnzb) { return (int)((nnzb + GATHER_BPG - 1) / GATHER_BPG); }
-// The blocks are taken in the order gorder (diagonal blocks first), so the
-// blocks of a wavefront loop about equally long.
+// FP64 mode takes the blocks in the order gorder (diagonal blocks first), so
+// the blocks of a wavefront loop about equally long; the tiled FP32 pass
+// leaves a diagonal block only a few chunk partials, and its gather
+// (gorder == nullptr) takes the blocks in index order.
 template <class R>
 __global__ void __launch_bounds__(TPB) k_gather_blocks(int nnzb, const int* __restrict__ gorder,
                                                        const int* __restrict__ bcptr, const R* __restrict__ tetH,
@@ -347,7 +349,7 @@ __global__ void __launch_bounds__(TPB) k_gather_blocks(int nnzb, const int* __re
     const long slot = ((long)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)) * GATHER_BPW + lane / 9;
     const int e = lane % 9;
     if (slot >= nnzb) return;
-    const int b = gorder[slot];
+    const int b = gorder ? gorder[slot] : (int)slot;
     double acc = 0.0;
     const int t1 = bcptr[b + 1];
     // unrolled so that several records are in flight (same summation order)
@@ -1954,7 +1956,7 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     STCHK(fipc_tm_end(c, TM_ELASTIC));
     STCHK(fipc_tm_begin(c, TM_GATHER));
     if (precision == 1)
-        k_gather_blocks<float><<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->gorder.p, c->tile_pptr.p,
+        k_gather_blocks<float><<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, nullptr, c->tile_pptr.p,
                                                                         c->tile_bpart.p, c->brow.p, c->col.p, c->m.p,
                                                                         nullptr, c->valf.p);
     else
