@@ -229,13 +229,16 @@ def run_gpu(args, variant):
     # blocks): its FP64 operations per launch from the committed rocprofv3
     # counters over its live event-timed duration
     flops = valu_flops()
-    if flops.get("k_elastic") and kt["elastic_tet_pass"]["us_avg"]:
+    # the FP32 mode's tet pass is the tiled kernel, the FP64 mode's k_elastic
+    tet = "k_elastic_tile" if precision == 1 else "k_elastic"
+    if flops.get(tet) and kt["elastic_tet_pass"]["us_avg"]:
         # the pass runs FP64 (SVD, projections, gradient) and, in FP32 mode,
-        # FP32 (the 12x12 contraction): its VALU fraction is the time each
-        # share would take at its own peak, over the measured duration
+        # FP32 (the 12x12 contraction, the sums over a chunk's tets): its VALU
+        # fraction is the time each share would take at its own peak, over
+        # the measured duration
         dur = kt["elastic_tet_pass"]["us_avg"] * 1e-6
-        f = flops["k_elastic"]
-        f32 = flops.get("k_elastic_fp32", 0) if precision == 1 else 0
+        f = flops[tet]
+        f32 = flops.get(tet + "_fp32", 0)
         tf = f / dur / 1e12
         frac = (f / (FP64_VALU_PEAK_TFLOPS * 1e12) + f32 / (FP32_VALU_PEAK_TFLOPS * 1e12)) / dur
         others["elastic_assembly"].update({"tet_pass_fp64_flops_per_launch": f, "tet_pass_fp64_tflops": round(tf, 2),
