@@ -764,20 +764,36 @@ __device__ __forceinline__ void amin_d(unsigned long long* a, double v)
     atomicMin(a, (unsigned long long)__double_as_longlong(v));
 }
 // narrow phase over the candidates (count read on the device, capped at cap:
-// an overflowing count makes the host redo the filter with more room)
+// an overflowing count makes the host redo the filter with more room).  A
+// pair's time of impact is the lesser of its two roundings (strict and
+// fused); the two run in different wavefronts (wavefront w takes the pairs
+// of group w / 2 of 64 in rounding w % 2), so each wavefront runs one root
+// isolation per pair: the minimum over (pair, rounding) is the minimum over
+// pairs of the lesser, bit for bit, with twice the wavefronts in flight.
+__device__ __forceinline__ bool ccd_slot(long t, int n, int* i, bool* fused)
+{
+    *i = (int)((t >> 7) * 64 + (t & 63));
+    *fused = (t >> 6) & 1;
+    return *i < n;
+}
 __global__ void __launch_bounds__(TPB) k_ccd_pt(const int* __restrict__ cnt, int cap, const int2* __restrict__ pairs,
                                                 const double* __restrict__ x, const double* __restrict__ p,
                                                 const int* __restrict__ bp, const int* __restrict__ bt, double eta,
                                                 unsigned long long* amin)
 {
-    int n = min(*cnt, cap);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int n = min(*cnt, cap);
+    const long nt = 2L * ((n + 63) / 64) * 64;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (long)gridDim.x * blockDim.x) {
+        int i;
+        bool fused;
+        if (!ccd_slot(t, n, &i, &fused)) continue;
         int2 pr = pairs[i];
         int q = bp[pr.y], t0 = bt[3 * pr.x], t1 = bt[3 * pr.x + 1], t2 = bt[3 * pr.x + 2];
         v3 P = ldv(x, q), T0 = ldv(x, t0), T1 = ldv(x, t1), T2 = ldv(x, t2);
         v3 dP = ldv(p, q), d0 = ldv(p, t0), d1 = ldv(p, t1), d2 = ldv(p, t2);
         double d2c = PT_dist2(P, T0, T1, T2, PT_type(P, T0, T1, T2));
-        double toc = pt_ccd(P, T0, T1, T2, dP, d0, d1, d2, eta, d2c);
+        double toc = fused ? ccd_fused::pt_ccd(P, T0, T1, T2, dP, d0, d1, d2, eta, d2c)
+                           : ccd_strict::pt_ccd(P, T0, T1, T2, dP, d0, d1, d2, eta, d2c);
         if (toc < 1.0) amin_d(amin, toc);
     }
 }
@@ -785,14 +801,19 @@ __global__ void __launch_bounds__(TPB) k_ccd_ee(const int* __restrict__ cnt, int
                                                 const double* __restrict__ x, const double* __restrict__ p,
                                                 const int* __restrict__ be, double eta, unsigned long long* amin)
 {
-    int n = min(*cnt, cap);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int n = min(*cnt, cap);
+    const long nt = 2L * ((n + 63) / 64) * 64;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (long)gridDim.x * blockDim.x) {
+        int i;
+        bool fused;
+        if (!ccd_slot(t, n, &i, &fused)) continue;
         int2 pr = pairs[i];
         int a0 = be[2 * pr.x], a1 = be[2 * pr.x + 1], b0 = be[2 * pr.y], b1 = be[2 * pr.y + 1];
         v3 A0 = ldv(x, a0), A1 = ldv(x, a1), B0 = ldv(x, b0), B1 = ldv(x, b1);
         v3 dA0 = ldv(p, a0), dA1 = ldv(p, a1), dB0 = ldv(p, b0), dB1 = ldv(p, b1);
         double d2c = EE_dist2(A0, A1, B0, B1, EE_type(A0, A1, B0, B1));
-        double toc = ee_ccd(A0, A1, B0, B1, dA0, dA1, dB0, dB1, eta, d2c);
+        double toc = fused ? ccd_fused::ee_ccd(A0, A1, B0, B1, dA0, dA1, dB0, dB1, eta, d2c)
+                           : ccd_strict::ee_ccd(A0, A1, B0, B1, dA0, dA1, dB0, dB1, eta, d2c);
         if (toc < 1.0) amin_d(amin, toc);
     }
 }
@@ -822,7 +843,7 @@ extern "C" int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* al
         CandCCDEE vee{c->x.p, c->sol.p, c->be.p, c->dHat, eta, PairSink{pc + 1, cap, c->ccd_pairs[1].p}};
         STCHK(run_query(c, 1, c->nbe, qedges, vee, TM_CCD, c->s2));
         STCHK(fipc_tm_begin(c, TM_NARROW, c->s2));
-        k_ccd_ee<<<stride_grid(cap), TPB, 0, c->s2>>>(pc + 1, cap, c->ccd_pairs[1].p, c->x.p, c->sol.p, c->be.p, eta,
+        k_ccd_ee<<<stride_grid(2L * cap), TPB, 0, c->s2>>>(pc + 1, cap, c->ccd_pairs[1].p, c->x.p, c->sol.p, c->be.p, eta,
                                                       c->amin.p + 1);
         HIPCHK(hipGetLastError());
         STCHK(fipc_tm_end(c, TM_NARROW, c->s2));
@@ -833,7 +854,7 @@ extern "C" int fipc_ccd_alpha(fipc_ctx* c, double eta, int inversion, double* al
         CandCCDPT vpt{c->x.p, c->sol.p, c->bp.p, c->bt.p, c->dHat, eta, PairSink{pc, cap, c->ccd_pairs[0].p}};
         STCHK(run_query(c, 0, c->nbt, tris, vpt, TM_CCD));
         STCHK(fipc_tm_begin(c, TM_NARROW));
-        k_ccd_pt<<<stride_grid(cap), TPB, 0, c->s>>>(pc, cap, c->ccd_pairs[0].p, c->x.p, c->sol.p, c->bp.p, c->bt.p, eta,
+        k_ccd_pt<<<stride_grid(2L * cap), TPB, 0, c->s>>>(pc, cap, c->ccd_pairs[0].p, c->x.p, c->sol.p, c->bp.p, c->bt.p, eta,
                                                      c->amin.p);
         HIPCHK(hipGetLastError());
         STCHK(fipc_tm_end(c, TM_NARROW));

@@ -125,6 +125,85 @@ def test_filters_match_oracle():
     assert g.check_collision() == o.check_collision() == 1
 
 
+def _ccd_brute_minimum(x, p, bp, be, bt, dHat, eta):
+    """Least time of impact of fipc_batch_ccd over every point-triangle and
+    edge-edge pair that passes the CCD pass's box and lower-bound filters
+    (compute_filter_3D_PT/EE, NEWTON_IPC.py:377-440), enumerated by brute
+    force: (point-triangle minimum, edge-edge minimum), 1.0 for none."""
+    def n2(a):
+        return a[..., 0] * a[..., 0] + a[..., 1] * a[..., 1] + a[..., 2] * a[..., 2]
+
+    def boxes(v):  # (n, k) vertex ids -> swept boxes
+        a, b = x[v], x[v] + p[v]
+        return np.minimum(a, b).min(axis=1), np.maximum(a, b).max(axis=1)
+
+    out = []
+    for kind in (0, 1):
+        if kind == 0:
+            va, vb = bp[:, None], bt
+        else:
+            va, vb = be, be
+        la, ha = boxes(va)
+        lb, hb = boxes(vb)
+        ii, jj = np.nonzero(np.all((la[:, None] < hb[None] + dHat) & (lb[None] - dHat < ha[:, None]), axis=2))
+        if kind == 1:
+            keep = ii < jj
+            ii, jj = ii[keep], jj[keep]
+        shared = (va[ii][:, :, None] == vb[jj][:, None, :]).any(axis=(1, 2))
+        ii, jj = ii[~shared], jj[~shared]
+        ids = np.concatenate([va[ii], vb[jj]], axis=1)  # (m, 4): P T0 T1 T2 / A0 A1 B0 B1
+        dn = np.sqrt(n2(p[ids]))
+        if kind == 0:
+            md = dn[:, 0] + np.maximum(np.maximum(dn[:, 1], dn[:, 2]), dn[:, 3])
+        else:
+            md = np.maximum(dn[:, 0], dn[:, 1]) + np.maximum(dn[:, 2], dn[:, 3])
+        ids, md = ids[md != 0], md[md != 0]
+        tf, df = ("PT_type", "PT_dist2") if kind == 0 else ("EE_type", "EE_dist2")
+        d2 = np.array([_lib.scalar(df, *x[q].ravel(), _lib.scalar(tf, *x[q].ravel())[0])[0] for q in ids])
+        keep = (1 - eta) * np.sqrt(d2) / md < 1
+        ids, d2 = ids[keep], np.ascontiguousarray(d2[keep])
+        n = len(ids)
+        if n == 0:
+            out.append(1.0)
+            continue
+        toc = np.zeros(n)
+        _lib.call("fipc_batch_ccd", kind, n, _lib.dptr(np.ascontiguousarray(x[ids].reshape(n, 12))),
+                  _lib.dptr(np.ascontiguousarray(p[ids].reshape(n, 12))), eta, _lib.dptr(d2), _lib.dptr(toc))
+        out.append(min(1.0, float(toc.min())))
+    return out
+
+
+def test_ccd_pass_equals_batch_minimum():
+    """The CCD pass (swept-box grid queries, filters, narrow phase with the
+    strict and fused roundings in separate wavefronts) gives, bit for bit,
+    the least fipc_batch_ccd time of impact over the brute-force pair set:
+    stacked cubes in contact, along the Newton direction scaled 10 and 30
+    times, whose pairs reach the root finding."""
+    sc = S.two_cubes(5)
+    o = OracleIPC(sc)
+    for _ in range(6):
+        o.step()
+    g = NewtonIPC(sc)
+    _sync(g, o, o.x.copy(), o.v.copy(), 6)
+    o.find_constraints()
+    o.assemble()
+    o.solve_system()
+    p0 = o.p.copy()
+    reached = 0
+    for scale in (10.0, 30.0):
+        p = np.ascontiguousarray(p0 * scale)
+        g.set_vector(1, p)
+        _, parts = g.compute_intersection_free_step_size(parts=True)
+        pairs = np.zeros(2, np.int32)
+        _lib.call("fipc_ccd_counts", g.ctx, _lib.iptr(pairs))
+        ref = _ccd_brute_minimum(o.x, p, o.bp.astype(np.int64), o.be.astype(np.int64), o.bt.astype(np.int64),
+                                 o.dHat, o.eta)
+        print(f"scale {scale}: pairs {pairs.tolist()}, parts {list(parts[:2])}, brute force {ref}")
+        assert parts[0] == ref[0] and parts[1] == ref[1]
+        reached += min(ref) < 1.0
+    assert reached >= 1
+
+
 @pytest.mark.parametrize("scene_fn,steps,amp", [(S.tiny_pair, 4, 0.001), (lambda: S.two_cubes(4), 12, 0.004)])
 def test_steps_match_oracle(scene_fn, steps, amp):
     """Whole time steps from the same pre-strained state (every tet away from
