@@ -174,6 +174,7 @@ struct fipc_ctx {
     // and the partials of r.u and r.r of two consecutive iterations
     DVec<double> cgs, part_g;
     DVec<float> cgsf;
+    DVec<float> dxf;  // an FP32 round's correction to the solution (zero between rounds)
     // prescribed rows of the step (fipc_begin_step_rows)
     DVec<int> trow;
     DVec<double> tval;

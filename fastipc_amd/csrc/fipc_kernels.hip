@@ -1097,7 +1097,9 @@ static inline int pcg_grid(int nv) { return std::min(nblk(nv), PCG_MAXB); }
 // is scal[0] from the round's start, whose residual its own test covered).
 //   beta = gamma / gamma_prev, alpha = gamma / (delta - beta gamma / alpha_prev)
 //   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = Minv r
-// and the partials of the new r.u and r.r go to part_g[it & 1].  The test of
+// and the partials of the new r.u and r.r go to part_g[it & 1].  x is the
+// solution (FP64 rounds) or, in the FP32 rounds, the round's correction in
+// FP32, which k_pcg_flush adds to the FP64 solution when the round ends.  The test of
 // an iteration's residual runs at the start of the next one (every block
 // decides it from the same partials; block 0 records it): on success
 // done_at = it - 1 and nothing more is written.  gamma and alpha of an
@@ -1109,7 +1111,7 @@ template <class V, class M>
 __global__ void __launch_bounds__(TPB) k_pcg_cg(int nv, double* scal, const unsigned char* __restrict__ dmask,
                                                 const M* __restrict__ Minv, V* __restrict__ u,
                                                 const V* __restrict__ w, double* __restrict__ qc,
-                                                V* __restrict__ p, V* __restrict__ s, double* __restrict__ x,
+                                                V* __restrict__ p, V* __restrict__ s, V* __restrict__ x,
                                                 V* __restrict__ r, const double* part_uw, int nuw, double* part_g,
                                                 int npg, int* state, int it, double tol2)
 {
@@ -1153,7 +1155,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_cg(int nv, double* scal, const unsi
             qcv[d] = qc[k + d];
             pv[d] = it == 0 ? 0.0 : (double)p[k + d];
             sv[d] = it == 0 ? 0.0 : (double)s[k + d];
-            xv[d] = x[k + d];
+            xv[d] = (double)x[k + d];
             rv[d] = (double)r[k + d];
         }
 #pragma unroll
@@ -1168,7 +1170,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_cg(int nv, double* scal, const unsi
             const V sn = (V)((wv[d] + qcv[d]) + beta * sv[d]);
             p[k + d] = pn;
             s[k + d] = sn;
-            x[k + d] = xv[d] + alpha * (double)pn;
+            x[k + d] = (V)(xv[d] + alpha * (double)pn);
             const V rnv = (V)(rv[d] - alpha * (double)sn);
             r[k + d] = rnv;
             ri[d] = (double)rnv;
@@ -1199,6 +1201,19 @@ __global__ void k_pcg_cg_last(double* scal, const double* part_g, int npg, int* 
     if (threadIdx.x == 0) {
         scal[4] = rr;
         if (rr <= tol2 * scal[2] || !(rr == rr)) state[0] = it - 1;
+    }
+}
+
+// end of an FP32 round: the solution takes the round's correction, which is
+// cleared for the next round
+__global__ void k_pcg_flush(int n, float* __restrict__ dx, double* __restrict__ x)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float d = dx[i];
+    if (d != 0.0f) {
+        x[i] += (double)d;
+        dx[i] = 0.0f;
     }
 }
 
@@ -1423,7 +1438,7 @@ extern "C" int fipc_destroy(fipc_ctx* c)
     release(c->keys_pp); release(c->keys_pe); release(c->keys_tmp); release(c->counters); release(c->cub_tmp); release(c->cub_tmp2); release(c->rows_tmp);
     release(c->nuniq); release(c->cdiag); release(c->Minv); release(c->r); release(c->z); release(c->pp); release(c->q); release(c->qc);
     release(c->Minvf); release(c->rf); release(c->zf); release(c->pf); release(c->qf);
-    release(c->cgs); release(c->cgsf); release(c->part_g);
+    release(c->cgs); release(c->cgsf); release(c->part_g); release(c->dxf);
     release(c->trow); release(c->tval);
     release(c->part_a); release(c->part_b); release(c->part_c); release(c->scal); release(c->pcg_state);
     release(c->x0flag);
@@ -2056,6 +2071,8 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
     const int nv = c->nv, nbv = nblk(nv), nsp = spmv_nparts(c), nu = pcg_grid(nv), chunk = PCG_CHUNK;
     const double tol_it = prec == 1 ? tol2 * (PCG_FP32_INNER * PCG_FP32_INNER) : tol2;
     V* sv = sizeof(V) == 4 ? (V*)(void*)c->cgsf.p : (V*)(void*)c->cgs.p;
+    // the iterations' x: the FP64 solution, or the FP32 round's correction
+    V* xs = sizeof(V) == 4 ? (V*)(void*)c->dxf.p : (V*)(void*)c->sol.p;
     int it = 0;
     state[0] = 0x7fffffff;
     state[1] = 0;
@@ -2095,7 +2112,7 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
         for (; it < end; it++) {
             STCHK(spmv<V>(c, prec, z, q, c->qc.p, c->part_c.p, c->pcg_state.p, it));
             STCHK(fipc_tm_begin(c, TM_PCGVEC));
-            k_pcg_cg<V, M><<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->dmask.p, Minv, z, q, c->qc.p, p, sv, c->sol.p, r,
+            k_pcg_cg<V, M><<<nu, TPB, 0, c->s>>>(nv, c->scal.p, c->dmask.p, Minv, z, q, c->qc.p, p, sv, xs, r,
                                                  c->part_c.p, nsp, c->part_g.p, nu, c->pcg_state.p, it, tol_it);
             STCHK(fipc_tm_end(c, TM_PCGVEC));
         }
@@ -2103,6 +2120,11 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
         STCHK(d2h_async(c, state, c->pcg_state.p, 8));
         STCHK(d2h_wait(c));
         if (state[0] != 0x7fffffff) break;
+    }
+    // the solution takes an FP32 round's correction
+    if (sizeof(V) == 4 && it > 0) {
+        k_pcg_flush<<<nblk(3 * nv), TPB, 0, c->s>>>(3 * nv, c->dxf.p, c->sol.p);
+        HIPCHK(hipGetLastError());
     }
     // the budget ran out: the last iteration's residual is tested here
     if (state[0] == 0x7fffffff && it > 0) {
@@ -2138,6 +2160,10 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
         STCHK(ensure(c->rf, (size_t)nv * 3)); STCHK(ensure(c->zf, (size_t)nv * 3));
         STCHK(ensure(c->pf, (size_t)nv * 3)); STCHK(ensure(c->qf, (size_t)nv * 3));
         STCHK(ensure(c->cgsf, (size_t)nv * 3));
+        if (c->dxf.cap < (size_t)nv * 3) {
+            STCHK(ensure(c->dxf, (size_t)nv * 3));
+            HIPCHK(hipMemsetAsync(c->dxf.p, 0, c->dxf.cap * sizeof(float), c->s));
+        }
     } else {
         STCHK(ensure(c->cgs, (size_t)nv * 3));
     }
