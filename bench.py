@@ -125,7 +125,8 @@ def make_sim(args, variant, sc):
 
     fp32, ws, inexact = VARIANTS[variant]
     sim = NewtonIPC(sc, precision=1 if fp32 else 0, pcg_tol=args.pcg_tol,
-                    pcg_tol_search=args.pcg_tol_search if inexact else None)
+                    pcg_tol_search=args.pcg_tol_search if inexact else None,
+                    certify=inexact and fp32 and not getattr(args, "no_certify", False))
     hook, net, warm_meta = None, None, None
     if ws == "learned":
         if not os.path.exists(WEIGHTS):
@@ -161,6 +162,7 @@ def run_gpu(args, variant):
             f"contacts {st.counts[-1] if st.counts else None}")
     _lib.call("fipc_sync", sim.ctx)
     phases0 = dict(sim.timers.total)
+    cert0 = sim.certified_solves
     t0 = time.perf_counter()
     pcg = []
     for i in range(args.steps):
@@ -175,6 +177,7 @@ def run_gpu(args, variant):
             raise RuntimeError("intersection after a step")
     _lib.call("fipc_sync", sim.ctx)
     wall = time.perf_counter() - t0
+    cert1 = sim.certified_solves
     # host wall time by phase of the timed steps (NewtonIPC's phase timers,
     # two levels deep), ms per step
     phases = {k: round((v - phases0.get(k, 0.0)) / args.steps * 1e3, 3) for k, v in sim.timers.total.items()
@@ -266,6 +269,7 @@ def run_gpu(args, variant):
     res = {
         "variant": variant,
         "warm_start_kept": accepted,
+        "certified_solves": cert1 - cert0,
         "warm_start_weights": warm_meta,
         "steps_per_s": args.steps / wall,
         "ms_per_step": wall / args.steps * 1e3,
@@ -396,6 +400,7 @@ def run_full_trajectory(args, variant, nsteps=200):
             raise RuntimeError("intersection after a step")
     _lib.call("fipc_sync", sim.ctx)
     wall = time.perf_counter() - t0
+    cert1 = sim.certified_solves
     counts = sim.find_constraints()
     rows = sim.constraints(sort=False)
     nslab = sim.nv - 16
@@ -458,6 +463,7 @@ def run_gpu_config0(args):
             raise RuntimeError("intersection after a step")
     _lib.call("fipc_sync", sim.ctx)
     wall = time.perf_counter() - t0
+    cert1 = sim.certified_solves
     sim.close()
     return {"steps_per_s": round((sc.steps - 1) / wall, 3), "newton_iters_per_step": round(newton / (sc.steps - 1), 2),
             "variant": "plain"}
@@ -509,6 +515,9 @@ def main():
     ap.add_argument("--pcg-tol", type=float, default=1e-6)
     ap.add_argument("--pcg-tol-search", type=float, default=2e-2,
                     help="PCG tolerance of the search directions of the +inexact variants")
+    ap.add_argument("--no-certify", action="store_true",
+                    help="solve the deciding direction of the FP32 +inexact variants to --pcg-tol even when "
+                         "its Newton test is certified earlier")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-full-trajectory", action="store_true",
                     help="skip the 200-step runs of the value's form and of plain FP64")
@@ -575,6 +584,10 @@ def main():
                    "newton_tol": "1e-2 dt bbox on max|p| (every variant)", "intersections": 0,
                    "pcg_rel_tol": args.pcg_tol,
                    "pcg_rel_tol_search_directions": args.pcg_tol_search if inexact else args.pcg_tol,
+                   # the deciding direction stops before pcg_rel_tol once the
+                   # exact direction provably passes the Newton test
+                   "newton_test_certificate": bool(inexact and fp32 and not args.no_certify),
+                   "certified_tests": g.get("certified_solves"),
                    "contacts_last_step": g["contacts_last_step"]},
         "same_run_variants": others,
         "speedup_vs_plain_fp64": round(g["steps_per_s"] / others["plain"]["steps_per_s"], 4) if "plain" in others else None,

@@ -51,6 +51,7 @@ _SIGS = {
     "fipc_assemble": [_VP, C.c_int],
     "fipc_solve": [_VP, _D, C.c_int, _IP, _P],
     "fipc_solve_refine": [_VP, _D, C.c_int, _IP, _P],
+    "fipc_solve_certify": [_VP, _D, C.c_int, _D, _IP, _P, _IP],
     "fipc_residual": [_VP, _P],
     "fipc_energy": [_VP, _P, _P],
     "fipc_ccd_alpha": [_VP, _D, C.c_int, _P, _P],

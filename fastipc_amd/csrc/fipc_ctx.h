@@ -235,6 +235,12 @@ struct fipc_ctx {
     // the last solve ended on an FP32-mode restart (r, z, p and the scalars
     // at its solution, FP64 residual) that a continuation can iterate from
     bool restart_ready = false;
+    // certified Newton test (fipc_solve_certify): while cert_bound > 0 a
+    // continuation stops once max|sol| + |b - A sol|_2 / cert_lam < cert_bound;
+    // cert_lam is half the least lumped mass of a free vertex (A >= M)
+    double cert_bound = 0.0, cert_lam = 0.0, cert_x = 0.0;
+    bool cert_hit = false;
+    int cert_fail = 0;
     KernelTimer tm[FIPC_NTIMERS];
     std::vector<hipEvent_t> event_pool;
     bool timing = false;

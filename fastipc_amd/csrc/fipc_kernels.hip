@@ -1686,6 +1686,12 @@ extern "C" int fipc_upload_mesh(fipc_ctx* c, int nv, const double* x0, const dou
         c->count[t] = 0;
     }
     c->cons_version++;  // no rows (and a new Dirichlet mask) from here
+    {
+        double mmin = 0.0;
+        for (int i = 0; i < nv; i++)
+            if (!dirichlet[i] && (mmin == 0.0 || m[i] < mmin)) mmin = m[i];
+        c->cert_lam = 0.5 * mmin;
+    }
     return build_pattern(c, nv, nt, tets);
 }
 
@@ -2064,9 +2070,29 @@ extern "C" int fipc_spmv(fipc_ctx* c, const double* x, double* y)
 
 // the iterations of one PCG round on vectors of type V (FP64 with the FP64
 // Hessian; FP32 in the FP32 Hessian's refinement rounds)
+// the certificate of the Newton test at the current solution: max|sol| and
+// |b - A sol|_2 from scal[4] (the FP64 residual of the restart just made);
+// true when max|sol| + |r|_2 / cert_lam < cert_bound
+static int cert_check(fipc_ctx* c, bool* ok)
+{
+    const int nbm = std::min(nblk(c->nv * 3), 1024);
+    k_maxabs<<<nbm, TPB, 0, c->s>>>(c->nv * 3, c->sol.p, c->part_a.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> hm(nbm);
+    double rr = 0.0;
+    STCHK(d2h_async(c, hm.data(), c->part_a.p, (size_t)nbm * 8));
+    STCHK(d2h_async(c, &rr, c->scal.p + 4, 8));
+    STCHK(d2h_wait(c));
+    double mx = 0.0;
+    for (double v : hm) mx = std::max(mx, v);
+    c->cert_x = mx;
+    *ok = rr == rr && mx + sqrt(rr) / c->cert_lam < c->cert_bound;
+    return FIPC_OK;
+}
+
 template <class V, class M>
 static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* q, int round, int budget,
-                     double tol2, int hint, int state[2], bool reuse, bool* init_only)
+                     double tol2, int hint, int state[2], bool reuse, bool* init_only, bool* cert_stop)
 {
     const int nv = c->nv, nbv = nblk(nv), nsp = spmv_nparts(c), nu = pcg_grid(nv), chunk = PCG_CHUNK;
     const double tol_it = prec == 1 ? tol2 * (PCG_FP32_INNER * PCG_FP32_INNER) : tol2;
@@ -2077,6 +2103,8 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
     state[0] = 0x7fffffff;
     state[1] = 0;
     *init_only = false;
+    *cert_stop = false;
+    const bool cert = c->cert_bound > 0.0 && sizeof(V) == 4 && round > 0;
     if (reuse) {
         // r, z, p and the scalars still hold the restart at the current
         // solution that ended the last solve; only the test changes
@@ -2102,6 +2130,19 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
             }
         }
     }
+    if (cert) {
+        // the FP64 residual of the current solution is in scal[4]: the
+        // certificate either holds (the round ends, the solve is done) or
+        // sets max|sol| for the estimates between chunks
+        bool ok = false;
+        STCHK(cert_check(c, &ok));
+        if (ok) {
+            c->cert_hit = true;
+            state[0] = -1;
+            return FIPC_OK;
+        }
+        if (it == 0) c->cert_fail++;
+    }
     // the host checks convergence between chunks of iterations; the first
     // chunk is sized from a previous solve of the same kind (hint), whose
     // count is usually close
@@ -2117,9 +2158,18 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
             STCHK(fipc_tm_end(c, TM_PCGVEC));
         }
         HIPCHK(hipGetLastError());
+        double rr = 0.0;
         STCHK(d2h_async(c, state, c->pcg_state.p, 8));
+        if (cert) STCHK(d2h_async(c, &rr, c->scal.p + 4, 8));
         STCHK(d2h_wait(c));
         if (state[0] != 0x7fffffff) break;
+        // the recursive residual suggests the certificate holds: end the
+        // round, whose successor's restart checks it with the FP64 residual
+        // (after a failed check, only at half the bound)
+        if (cert && c->cert_x + sqrt(rr) / c->cert_lam < c->cert_bound * (c->cert_fail > 1 ? 0.5 : 1.0)) {
+            *cert_stop = true;
+            break;
+        }
     }
     // the solution takes an FP32 round's correction
     if (sizeof(V) == 4 && it > 0) {
@@ -2127,7 +2177,7 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
         HIPCHK(hipGetLastError());
     }
     // the budget ran out: the last iteration's residual is tested here
-    if (state[0] == 0x7fffffff && it > 0) {
+    if (state[0] == 0x7fffffff && it > 0 && !*cert_stop) {
         k_pcg_cg_last<<<1, TPB, 0, c->s>>>(c->scal.p, c->part_g.p, nu, c->pcg_state.p, it, tol_it);
         HIPCHK(hipGetLastError());
         STCHK(d2h_async(c, state, c->pcg_state.p, 8));
@@ -2168,9 +2218,10 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
         STCHK(ensure(c->cgs, (size_t)nv * 3));
     }
     STCHK(ensure(c->part_g, 4 * PCG_MAXB));
-    const int rounds = prec == 1 ? 4 : 1;
+    // a certified continuation may end rounds early: two more rounds
+    const int rounds = prec == 1 ? (c->cert_bound > 0.0 ? 6 : 4) : 1;
     int total = 0, state[2] = {0, 0};
-    bool converged = false, init_only = false;
+    bool converged = false, init_only = false, cert_stop = false;
     // a continuation in FP32 mode starts from the restart that ended the
     // solve it continues, when there is one (its FP64 residual is current)
     bool reuse = first_round > 0 && prec == 1 && c->restart_ready;
@@ -2193,15 +2244,16 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
         const bool ru = reuse && round == first_round;
         if (prec == 1)
             STCHK((pcg_round<float, float>(c, prec, c->Minvf.p, c->rf.p, c->zf.p, c->pf.p, c->qf.p, round, budget,
-                                           tol2, hint, state, ru, &init_only)));
+                                           tol2, hint, state, ru, &init_only, &cert_stop)));
         else
             STCHK((pcg_round<double, double>(c, prec, c->Minv.p, c->r.p, c->z.p, c->pp.p, c->q.p, round, budget,
-                                             tol2, hint, state, ru, &init_only)));
+                                             tol2, hint, state, ru, &init_only, &cert_stop)));
         if (state[0] == -1 && round > 0) { converged = true; break; }
         int ran = state[0] == -1 ? 0 : state[1];
         total += ran;
         if (round == 0 && ran > 0) c->pcg_hint = ran;
-        if (round == first_round && first_round > 0 && ran > 0) c->refine_hint = ran;
+        if (round == first_round && first_round > 0 && ran > 0 && !cert_stop) c->refine_hint = ran;
+        if (cert_stop) continue;                  // the next round's restart checks the certificate
         if (state[0] == 0x7fffffff) break;        // iteration budget spent
         if (prec != 1 || state[0] == -1) { converged = true; break; }
         // FP32 iterations converged: the next round checks the FP64 residual
@@ -2212,7 +2264,7 @@ static int solve_rounds(fipc_ctx* c, int first_round, double tol, int maxit, int
     if (!converged && state[0] != 0x7fffffff) {
         int chk[2] = {0, 0};
         STCHK((pcg_round<float, float>(c, prec, c->Minvf.p, c->rf.p, c->zf.p, c->pf.p, c->qf.p,
-                                       first_round + rounds, 0, tol2, 0, chk, false, &init_only)));
+                                       first_round + rounds, 0, tol2, 0, chk, false, &init_only, &cert_stop)));
         converged = chk[0] == -1;
     }
     // a solve that ended on a restart whose FP64 residual met the tolerance
@@ -2258,6 +2310,26 @@ extern "C" int fipc_solve_refine(fipc_ctx* c, double tol, int maxit, int* iters,
     if (!c || !c->nv || c->solve_version != c->sys_version || c->cv_version != c->cons_version)
         return FIPC_ERR_STATE;
     return solve_rounds(c, 1, tol, maxit, iters, relres);
+}
+
+// fipc_solve_refine that stops as soon as the exact direction provably passes
+// the Newton test max|p| < bound: A >= M (the lumped mass; the elastic and
+// barrier Hessians are projected to PSD), so |p* - p|_inf <= |p* - p|_2 <=
+// |b - A p|_2 / min m, halved for the FP32 storage of A.  Checked with the
+// FP64 residual at round restarts, which the recursive residual triggers.
+extern "C" int fipc_solve_certify(fipc_ctx* c, double tol, int maxit, double bound, int* iters, double* relres,
+                                  int* certified)
+{
+    if (!c || !c->nv || c->solve_version != c->sys_version || c->cv_version != c->cons_version)
+        return FIPC_ERR_STATE;
+    if (!(bound > 0.0)) return FIPC_ERR_ARG;
+    c->cert_hit = false;
+    c->cert_fail = 0;
+    c->cert_bound = c->precision == 1 && c->cert_lam > 0.0 ? bound : 0.0;
+    const int rc = solve_rounds(c, 1, tol, maxit, iters, relres);
+    c->cert_bound = 0.0;
+    if (certified) *certified = c->cert_hit ? 1 : 0;
+    return rc;
 }
 
 // ---- energy ----

@@ -66,7 +66,7 @@ class StepStats:
 class NewtonIPC:
     def __init__(self, scene, precision: int = 0, pcg_tol: float = 1e-6, pcg_maxit: int = 20000,
                  max_newton: int = 200, max_halvings: int = 60, inversion: bool = True, timing: bool = False,
-                 pcg_tol_search: float | None = None):
+                 pcg_tol_search: float | None = None, certify: bool = False):
         self.so = _lib.load()
         self.scene = scene
         self.setup = s = SceneSetup(scene)
@@ -75,6 +75,11 @@ class NewtonIPC:
         if pcg_tol_search is not None and not pcg_tol_search >= pcg_tol:
             raise ValueError("pcg_tol_search must be at least pcg_tol")
         self.pcg_tol_search = pcg_tol_search
+        # with inexact search directions: the deciding direction's
+        # continuation stops once the exact direction provably passes the
+        # Newton test (fipc_solve_certify; FP32 mode)
+        self.certify = certify
+        self.certified_solves = 0
         self.pcg_maxit = pcg_maxit
         self.last_pcg_iters = 0
         self.last_solve_converged = True
@@ -223,16 +228,25 @@ class NewtonIPC:
         self.last_solve_converged = code == 0
         return it.value, rel.value
 
-    def refine_solve(self, tol: float | None = None):
+    def refine_solve(self, tol: float | None = None, bound: float | None = None):
         """Continues the last solve of the current system to ``tol``
-        (default ``pcg_tol``) from its solution."""
+        (default ``pcg_tol``) from its solution; with ``bound``, stops as soon
+        as max|p*| < bound is certified for the exact direction p*
+        (fipc_solve_certify), which counts as converged."""
         it = C.c_int()
         rel = C.c_double()
         tol = self.pcg_tol if tol is None else tol
         maxit = max(self.pcg_maxit - self.last_pcg_iters, 1)
-        code = self.so.fipc_solve_refine(self.ctx, tol, maxit, C.byref(it), C.byref(rel))
+        if bound is None:
+            code = self.so.fipc_solve_refine(self.ctx, tol, maxit, C.byref(it), C.byref(rel))
+            name = "fipc_solve_refine"
+        else:
+            cert = C.c_int()
+            code = self.so.fipc_solve_certify(self.ctx, tol, maxit, bound, C.byref(it), C.byref(rel), C.byref(cert))
+            name = "fipc_solve_certify"
+            self.certified_solves += cert.value
         if code not in (0, 5):
-            raise _lib.FipcError("fipc_solve_refine", code)
+            raise _lib.FipcError(name, code)
         self.last_solve_converged = code == 0
         return it.value, rel.value
 
@@ -304,9 +318,10 @@ class NewtonIPC:
                     it, rel = self.solve_system(f * dt, self.pcg_tol_search)
                 res = self.output_residual()
                 if self.pcg_tol_search is not None and res < self.setup.newton_tol and self.last_solve_converged:
-                    # the direction that decides convergence is solved to pcg_tol
+                    # the direction that decides convergence is solved to
+                    # pcg_tol, or until its test is certified
                     with T("solve"):
-                        it2, rel = self.refine_solve()
+                        it2, rel = self.refine_solve(bound=self.setup.newton_tol if self.certify else None)
                     it += it2
                     res = self.output_residual()
                 st.pcg_iters.append(it)

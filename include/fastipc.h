@@ -210,6 +210,16 @@ int fipc_solve(fipc_ctx* c, double tol, int maxit, int* iters, double* relres);
  * tighten only the solve whose direction decides convergence.  No
  * counterpart in the reference, whose direct solve is exact. */
 int fipc_solve_refine(fipc_ctx* c, double tol, int maxit, int* iters, double* relres);
+/* fipc_solve_refine that ends early when the exact direction p* provably
+ * passes the Newton test max|p*| < bound: since A >= M (lumped mass; the
+ * elastic and barrier Hessians are projected to PSD), max|p*| <= max|p| +
+ * |b - A p|_2 / (m_min / 2) with m_min the least lumped mass of a free
+ * vertex (halved for the FP32 storage of A).  Checked with the FP64
+ * residual of a restart; *certified = 1 when it held.  FP32 mode only (in
+ * FP64 mode it is fipc_solve_refine).  No counterpart in the reference,
+ * whose direct solve is exact. */
+int fipc_solve_certify(fipc_ctx* c, double tol, int maxit, double bound, int* iters, double* relres,
+                       int* certified);
 /* output_residual (NEWTON_IPC.py:805-812): max |p_i| */
 int fipc_residual(fipc_ctx* c, double* res);
 /* compute_energy (NEWTON_IPC.py:514-542); parts[3] = inertia, elastic, barrier (may be NULL) */

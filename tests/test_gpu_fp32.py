@@ -247,6 +247,32 @@ def test_inexact_search_directions_steps(prec):
         assert err <= 1e-6 * sc.bbox_diag, (f, err)
 
 
+def test_certified_newton_test_keeps_the_trajectory():
+    """With inexact search directions in FP32 mode, the deciding direction's
+    continuation may stop once max|p*| < tol is certified for the exact
+    direction (max|p| + |b - A p|_2 / (m_min / 2), A >= M).  A certified
+    direction is never applied (the step ends), so on a 10x16x10 compressed
+    slab with moving plates, 12 steps with and without the certificate give
+    the same Newton iterations and the same positions bit for bit, with at
+    least one certified test and fewer PCG iterations."""
+    sc = S.compressed_slab(10, 16, 10, steps=12)
+    runs = {}
+    for cert in (False, True):
+        g = NewtonIPC(sc, precision=1, pcg_tol_search=2e-2, certify=cert)
+        newton, pcg = [], 0
+        for _ in range(12):
+            st = g.step()
+            assert st.converged
+            newton.append(st.newton_iters)
+            pcg += sum(st.pcg_iters)
+        runs[cert] = (g.x.copy(), newton, pcg, g.certified_solves)
+    (x0, n0, p0, _), (x1, n1, p1, c1) = runs[False], runs[True]
+    print(f"newton {n1}, pcg {p0} -> {p1}, certified tests {c1}")
+    assert n0 == n1
+    assert np.array_equal(x0, x1)
+    assert c1 >= 1 and p1 < p0
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_residual_is_max_abs_direction(prec):
     """output_residual (max |p|, NEWTON_IPC.py:805-812) equals the maximum of
