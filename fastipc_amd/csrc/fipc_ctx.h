@@ -241,6 +241,7 @@ struct fipc_ctx {
     double cert_bound = 0.0, cert_lam = 0.0, cert_x = 0.0;
     bool cert_hit = false;
     int cert_fail = 0;
+    int cert_hint = 0;  // iterations the last certificate's round took
     KernelTimer tm[FIPC_NTIMERS];
     std::vector<hipEvent_t> event_pool;
     bool timing = false;

@@ -2146,9 +2146,16 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
     // the host checks convergence between chunks of iterations; the first
     // chunk is sized from a previous solve of the same kind (hint), whose
     // count is usually close
-    int first = std::max(chunk, hint - PCG_FIRST_SHORT);
+    // (towards a certificate: from the iterations the last one took)
+    int first = cert ? std::max(chunk / 2, c->cert_hint - PCG_FIRST_SHORT) : std::max(chunk, hint - PCG_FIRST_SHORT);
+    // with a certificate to reach, the next chunk is sized from the
+    // residual's decay over the last one, to end about where the estimate
+    // crosses the bound (at most chunk iterations)
+    int next = chunk;
+    double rr_last = -1.0;
     while (it < budget) {
-        int end = std::min(budget, it + (it == 0 ? first : chunk));
+        const int it0 = it;
+        int end = std::min(budget, it + (it == 0 ? first : next));
         // single-reduction iterations: the product of u = z, then one update
         for (; it < end; it++) {
             STCHK(spmv<V>(c, prec, z, q, c->qc.p, c->part_c.p, c->pcg_state.p, it));
@@ -2166,9 +2173,21 @@ static int pcg_round(fipc_ctx* c, int prec, const M* Minv, V* r, V* z, V* p, V* 
         // the recursive residual suggests the certificate holds: end the
         // round, whose successor's restart checks it with the FP64 residual
         // (after a failed check, only at half the bound)
-        if (cert && c->cert_x + sqrt(rr) / c->cert_lam < c->cert_bound * (c->cert_fail > 1 ? 0.5 : 1.0)) {
+        const double bound = c->cert_bound * (c->cert_fail > 1 ? 0.5 : 1.0);
+        if (cert && c->cert_x + sqrt(rr) / c->cert_lam < bound) {
             *cert_stop = true;
+            c->cert_hint = it;
             break;
+        }
+        if (cert) {
+            const double target = (bound - c->cert_x) * c->cert_lam;  // the |r|_2 that passes
+            const int len = it - it0;
+            next = chunk;
+            if (target > 0.0 && rr_last > 0.0 && rr < rr_last && len > 0) {
+                const double n = ceil(log(target / sqrt(rr)) / (0.5 * log(rr / rr_last) / len));
+                next = n >= chunk ? chunk : (n < 1.0 ? 1 : (int)n);
+            }
+            rr_last = rr;
         }
     }
     // the solution takes an FP32 round's correction
