@@ -31,8 +31,7 @@ assembly with the FP64 VALU fraction of its per-tet pass, CCD with a probe of
 its narrow phase, broad phase), the whole 200-step trajectory of the value's
 form and of plain FP64 (full_trajectory), and a CPU baseline: the C oracle
 (OpenMP on every host core, a direct sparse solve per Newton iteration) on
-configs[0], two stacked cubes for 50 steps, beside the plain GPU path on the
-same scene.
+two stacked cubes for 50 steps, beside the plain GPU path on the same scene.
 """
 from __future__ import annotations
 
@@ -75,7 +74,7 @@ def measured_traffic(mode):
 
 
 def algorithmic_bytes(sim, precision):
-    """Compulsory HBM bytes per launch of the kernel classes (see DESIGN.md)."""
+    """Compulsory HBM bytes per launch of the kernel classes."""
     s = sim.setup
     nv, nt = sim.nv, len(s.tets)
     nu, nl = C.c_int(), C.c_int()
@@ -421,9 +420,9 @@ def run_full_trajectory(args, variant, nsteps=200):
 
 def run_cpu_baseline(args):
     """The oracle (strict-IEEE C, OpenMP on every host core, CSR + a direct
-    sparse solve per Newton iteration: the reference's structure) on
-    configs[0] of the benchmark definition: two stacked 10^3-vertex cubes
-    dropped on a ground box, 50 steps; steps/s over steps 2..50 (the first
+    sparse solve per Newton iteration: the reference's structure) on the
+    two-stacked-cubes workload: two 10^3-vertex cubes dropped on a ground
+    box, 50 steps; steps/s over steps 2..50 (the first
     is excluded, as the reference's Timer_Print does)."""
     from fastipc_amd import scene as S
     from oracle import lib as OL
@@ -439,13 +438,13 @@ def run_cpu_baseline(args):
     wall = time.perf_counter() - t0
     return {"value": round((sc.steps - 1) / wall, 4), "unit": "steps/s", "cores": OL.lib().orc_num_threads(),
             "kind": "port",
-            "sample": f"two stacked cubes (configs[0]: {sc.n_verts} verts, {sc.n_tets} tets), {sc.steps} steps, "
+            "sample": f"two stacked cubes ({sc.n_verts} verts, {sc.n_tets} tets), {sc.steps} steps, "
                       f"steps 2-{sc.steps} timed ({wall:.1f} s, {newton / (sc.steps - 1):.2f} Newton iterations "
                       f"per step), direct solver {SOLVER}; a different, far smaller scene than the GPU value's"}
 
 
 def run_gpu_config0(args):
-    """The plain FP64 GPU path on the CPU baseline's scene,
+    """The plain FP64 GPU path on the CPU baseline's scene (two stacked cubes),
     counted the same way, for a same-scene comparison."""
     from fastipc_amd import _lib
     from fastipc_amd import scene as S

@@ -137,11 +137,12 @@ def test_compressed_slab_full_size_into_compression(variant):
 
 @pytest.mark.parametrize("cfg,steps", [(0, 50), (1, 100), (2, 150)])
 def test_benchmark_scenes_into_contact(cfg, steps):
-    """Configs 0-2 at their sizes and lengths (config 2: 150 steps), run into
+    """The stacked cubes, the cube grid and the cube column at their sizes and
+    lengths (the column: 150 steps), run into
     contact: after the GPU steps the oracle finds the same constraint sets,
     energy and gradient, and no intersection; rows between two different
     bodies (not the static ground or walls) are asserted for the cube
-    scenes.  Steps of configs 1 and 2 may end stalled: an edge pair's
+    scenes.  Steps of the cube grid and the column may end stalled: an edge pair's
     closest-feature switch adds a barrier term that the np.unique merge of
     point-edge rows had counted once, so the energy jumps along the
     direction and the reference's line search would halve alpha without end

@@ -317,7 +317,7 @@ def test_energy_reuse_matches_full_evaluation():
 
 
 def test_fp32_per_step_error_against_fp64_on_cube_column():
-    """The FP32-Hessian form on the 200-cube column (configs[2], 343k tets)
+    """The FP32-Hessian form on the 200-cube column (343k tets)
     against FP64, step by step from the same state (the FP64 run's): after
     each step the FP32 step's positions, its incremental potential and the
     norm of its gradient (both evaluated by the FP64 oracle at the end
