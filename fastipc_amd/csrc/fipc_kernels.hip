@@ -334,9 +334,8 @@ constexpr int GATHER_BPW = 7;                       // blocks per wavefront
 constexpr int GATHER_BPG = GATHER_BPW * (TPB / 64); // blocks per workgroup
 static inline int gather_grid(long nnzb) { return (int)((nnzb + GATHER_BPG - 1) / GATHER_BPG); }
 // FP64 mode takes the blocks in the order gorder (diagonal blocks first), so
-// the blocks of a wavefront loop about equally long; the tiled FP32 pass
-// leaves a diagonal block only a few chunk partials, and its gather
-// (gorder == nullptr) takes the blocks in index order.
+// the blocks of a wavefront loop about equally long (gorder == nullptr: index
+// order).
 template <class R>
 __global__ void __launch_bounds__(TPB) k_gather_blocks(int nnzb, const int* __restrict__ gorder,
                                                        const int* __restrict__ bcptr, const R* __restrict__ tetH,
@@ -358,6 +357,39 @@ __global__ void __launch_bounds__(TPB) k_gather_blocks(int nnzb, const int* __re
     if ((e == 0 || e == 4 || e == 8) && col[b] == brow[b]) acc += m[brow[b]];
     if (val) val[(size_t)b * 9 + e] = acc;
     if (valf) valf[(size_t)b * 9 + e] = (float)acc;
+}
+
+// the FP32 tiled pass's gather: as k_gather_blocks in index order, but each
+// group of nine lanes sums two blocks (wavefront w takes blocks 14 w .. +13,
+// group g the blocks 14 w + g and 14 w + 7 + g) so that twice the records are
+// in flight; each block's partials are still summed in chunk order
+__global__ void __launch_bounds__(TPB) k_gather_parts(int nnzb, const int* __restrict__ pptr,
+                                                      const float* __restrict__ part, const int* __restrict__ brow,
+                                                      const int* __restrict__ col, const double* __restrict__ m,
+                                                      float* __restrict__ valf)
+{
+    const int lane = threadIdx.x & 63;
+    if (lane >= 9 * GATHER_BPW) return;
+    const long w = (long)blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
+    const long b0 = w * 2 * GATHER_BPW + lane / 9, b1 = b0 + GATHER_BPW;
+    const int e = lane % 9;
+    if (b0 >= nnzb) return;
+    const bool two = b1 < nnzb;
+    const int s0 = pptr[b0], t0 = pptr[b0 + 1];
+    const int s1 = two ? pptr[b1] : 0, t1 = two ? pptr[b1 + 1] : 0;
+    double a0 = 0.0, a1 = 0.0;
+    const int n0 = t0 - s0, n1 = t1 - s1, n = n0 > n1 ? n0 : n1;
+#pragma unroll 2
+    for (int k = 0; k < n; k++) {
+        if (k < n0) a0 += (double)part[(size_t)(s0 + k) * 9 + e];
+        if (k < n1) a1 += (double)part[(size_t)(s1 + k) * 9 + e];
+    }
+    if ((e == 0 || e == 4 || e == 8) && col[b0] == brow[b0]) a0 += m[brow[b0]];
+    valf[(size_t)b0 * 9 + e] = (float)a0;
+    if (two) {
+        if ((e == 0 || e == 4 || e == 8) && col[b1] == brow[b1]) a1 += m[brow[b1]];
+        valf[(size_t)b1 * 9 + e] = (float)a1;
+    }
 }
 
 // the gradient in two parts around the contact assembly, which runs on a
@@ -1977,9 +2009,8 @@ extern "C" int fipc_assemble(fipc_ctx* c, int precision)
     STCHK(fipc_tm_end(c, TM_ELASTIC));
     STCHK(fipc_tm_begin(c, TM_GATHER));
     if (precision == 1)
-        k_gather_blocks<float><<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, nullptr, c->tile_pptr.p,
-                                                                        c->tile_bpart.p, c->brow.p, c->col.p, c->m.p,
-                                                                        nullptr, c->valf.p);
+        k_gather_parts<<<(c->nnzb + 2 * GATHER_BPG - 1) / (2 * GATHER_BPG), TPB, 0, c->s>>>(
+            c->nnzb, c->tile_pptr.p, c->tile_bpart.p, c->brow.p, c->col.p, c->m.p, c->valf.p);
     else
         k_gather_blocks<double><<<gather_grid(c->nnzb), TPB, 0, c->s>>>(c->nnzb, c->gorder.p, c->bcptr.p, c->tetH.p, c->brow.p,
                                                                          c->col.p, c->m.p, c->val.p, nullptr);
