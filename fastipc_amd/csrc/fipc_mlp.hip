@@ -218,34 +218,41 @@ __global__ void k_gather_batch(int B, long nds, unsigned long long seed, int nin
     for (int k = 0; k < nout; k++) T[(long)i * nout + k] = dsT[r * nout + k];
 }
 
-#ifndef FI_ROWS
-#define FI_ROWS 64
+#ifndef FI_RT
+#define FI_RT 4
 #endif
-// 4 waves per 64 rows: wave w takes features 64 (w % 4) .. +64 of rows 64 (w / 4) .. +64
-constexpr int FI_THREADS = 4 * (FI_ROWS / 64) * 64;
+#ifndef FI_ROWS
+#define FI_ROWS (32 * FI_RT)
+#endif
+// a wave takes 64 output features (two 32-feature tiles) of FI_RT 32-row
+// tiles: wave w takes features 64 (w % 4) .. +64 of rows 32 FI_RT (w / 4) ..
+constexpr int FI_WROWS = 32 * FI_RT;
+static_assert(FI_ROWS % FI_WROWS == 0, "rows per block");
+constexpr int FI_THREADS = 4 * (FI_ROWS / FI_WROWS) * 64;
 #define FI_W 256
 #define FI_LDS (FI_W + 8)
 // D[feature][row] += W[feature][k..k+15] . H[row][k..k+15] over the layer's
-// fan-in KIN for NF 32-feature tiles from f0 and two 32-row tiles from rbase.
+// fan-in KIN for NF 32-feature tiles from f0 and RT 32-row tiles from rbase.
 // The A operand is a weight fragment (8 consecutive k of one output feature,
 // one 16-byte load from the transposed BF16 weights), the B operand an
 // activation fragment (8 consecutive k of one row, one 16-byte LDS read);
 // the next k step's weight fragments are loaded before this step's products.
-template <int NF>
+template <int NF, int RT>
 __device__ __forceinline__ void fi_step(const bf16x8 (&a)[NF], const __bf16 (*H)[FI_LDS], int rbase, int lane, int k0,
-                                        floatx16 (&acc)[NF][2])
+                                        floatx16 (&acc)[NF][RT])
 {
-    bf16x8 b0 = *(const bf16x8*)&H[rbase + (lane & 31)][k0 + (lane >> 5) * 8];
-    bf16x8 b1 = *(const bf16x8*)&H[rbase + 32 + (lane & 31)][k0 + (lane >> 5) * 8];
+    bf16x8 b[RT];
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
-        acc[f][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[f], b0, acc[f][0], 0, 0, 0);
-        acc[f][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[f], b1, acc[f][1], 0, 0, 0);
-    }
+    for (int rt = 0; rt < RT; rt++) b[rt] = *(const bf16x8*)&H[rbase + rt * 32 + (lane & 31)][k0 + (lane >> 5) * 8];
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++)
+            acc[f][rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[f], b[rt], acc[f][rt], 0, 0, 0);
 }
-template <int KIN, int NF>
+template <int KIN, int NF, int RT>
 __device__ __forceinline__ void fi_layer(const __bf16* __restrict__ W, const __bf16 (*H)[FI_LDS], int f0, int rbase,
-                                         int lane, floatx16 (&acc)[NF][2])
+                                         int lane, floatx16 (&acc)[NF][RT])
 {
     static_assert(KIN == 16 || KIN % 32 == 0, "fan-in");
     const __bf16* wp[NF];
@@ -256,7 +263,7 @@ __device__ __forceinline__ void fi_layer(const __bf16* __restrict__ W, const __b
         a[f] = *(const bf16x8*)wp[f];
     }
     if (KIN == 16) {
-        fi_step<NF>(a, H, rbase, lane, 0, acc);
+        fi_step<NF, RT>(a, H, rbase, lane, 0, acc);
         return;
     }
     // two k steps per pass in alternating registers: the fragments of the
@@ -265,11 +272,11 @@ __device__ __forceinline__ void fi_layer(const __bf16* __restrict__ W, const __b
     for (int k0 = 0; k0 < KIN; k0 += 32) {
 #pragma unroll
         for (int f = 0; f < NF; f++) an[f] = *(const bf16x8*)(wp[f] + k0 + 16);
-        fi_step<NF>(a, H, rbase, lane, k0, acc);
+        fi_step<NF, RT>(a, H, rbase, lane, k0, acc);
         const int kn = k0 + 32 < KIN ? k0 + 32 : KIN - 16;
 #pragma unroll
         for (int f = 0; f < NF; f++) a[f] = *(const bf16x8*)(wp[f] + kn);
-        fi_step<NF>(an, H, rbase, lane, k0 + 16, acc);
+        fi_step<NF, RT>(an, H, rbase, lane, k0 + 16, acc);
     }
 }
 __device__ __forceinline__ unsigned int bf16_bits(float v)
@@ -280,7 +287,8 @@ __device__ __forceinline__ unsigned int bf16_bits(float v)
 // holds features f + 8 (j / 4) + 4 (l / 32) + j % 4 of row (l % 32) of each
 // tile, so each group of four j is four consecutive features of one row:
 // one 8-byte LDS store
-__device__ __forceinline__ void fi_silu_store(const floatx16 (&acc)[2][2], const float* __restrict__ bias,
+template <int RT>
+__device__ __forceinline__ void fi_silu_store(const floatx16 (&acc)[2][RT], const float* __restrict__ bias,
                                               __bf16 (*H)[FI_LDS], int f0, int rbase, int lane)
 {
 #pragma unroll
@@ -290,7 +298,7 @@ __device__ __forceinline__ void fi_silu_store(const floatx16 (&acc)[2][2], const
             const int m = f0 + ft * 32 + 8 * g + 4 * (lane >> 5);
             const float b0 = bias[m], b1 = bias[m + 1], b2 = bias[m + 2], b3 = bias[m + 3];
 #pragma unroll
-            for (int rt = 0; rt < 2; rt++) {
+            for (int rt = 0; rt < RT; rt++) {
                 float z0 = acc[ft][rt][4 * g] + b0, z1 = acc[ft][rt][4 * g + 1] + b1;
                 float z2 = acc[ft][rt][4 * g + 2] + b2, z3 = acc[ft][rt][4 * g + 3] + b3;
                 uint2 pk;
@@ -302,27 +310,28 @@ __device__ __forceinline__ void fi_silu_store(const floatx16 (&acc)[2][2], const
 }
 // ---------------------------------------------------------------------------
 // Fused inference for the warm-start shape (16 -> 4 x 256 SiLU -> 3): a block
-// of 4 waves takes 64 rows through all five layers; the activations stay in
-// LDS as BF16 (the products' input precision) and never touch HBM.  Weights
-// are read from L2 as BF16 transposed per layer, Wt[n][k] (the last layer
-// padded to 32 rows of zeros).  Wave w owns output features 64 w .. +64 (two
-// 32-feature MFMA tiles) of the block's rows (two 32-row tiles): each weight
-// fragment feeds two MFMAs, each activation fragment two.  At most 128
-// registers per lane: 4 waves per SIMD, four blocks per CU (33 KB activation
-// tile each), whose layer barriers fall at different times, so one block's
-// products overlap another's SiLU epilogue (128-row blocks of 8 waves, two
-// per CU: 229 against 201 us on 200k rows).  The accumulation order over k is
-// the generic GEMM's.
+// takes FI_ROWS rows through all five layers; the activations stay in LDS as
+// BF16 (the products' input precision) and never touch HBM.  Weights are read
+// from L2 as BF16 transposed per layer, Wt[n][k] (the last layer padded to 32
+// rows of zeros).  In the hidden layers wave w owns output features
+// 64 (w % 4) .. +64 (two 32-feature MFMA tiles) of its FI_RT 32-row tiles:
+// each weight fragment feeds FI_RT MFMAs, each activation fragment two.  The
+// output layer's 32-row tiles are shared out over all the block's waves.
+// The weight fragments come from L1/L2, so rows per wave set the loads per
+// product: 200k rows take 199 us with 2 row tiles (108 VGPRs, 4 waves per
+// SIMD), 159 us with 3, 151 us with 4 (192 VGPRs, 2 waves per SIMD, two
+// 128-row blocks per CU), all with the same bits; 5 or more row tiles fit only
+// one wave per SIMD.  The accumulation order over k is the generic GEMM's.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(FI_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(FI_THREADS) __attribute__((amdgpu_waves_per_eu(FI_RT <= 2 ? 4 : 2)))
 k_mlp_infer(int n, const float* __restrict__ X, const __bf16* __restrict__ Wt, const float* __restrict__ params,
             const long* __restrict__ boff, float* __restrict__ Y)
 {
     __shared__ __attribute__((aligned(16))) __bf16 H[FI_ROWS][FI_LDS];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int f0 = (w & 3) * 64, rbase = (w >> 2) * 64;
+    const int f0 = (w & 3) * 64, rbase = (w >> 2) * FI_WROWS;
     const long r0 = (long)blockIdx.x * FI_ROWS;
-    // input tile: 128 rows x 16 features
+    // input tile: FI_ROWS rows x 16 features
     for (int e = tid; e < FI_ROWS * 16; e += blockDim.x) {
         int r = e >> 4, k = e & 15;
         long gr = r0 + r;
@@ -331,38 +340,32 @@ k_mlp_infer(int n, const float* __restrict__ X, const __bf16* __restrict__ Wt, c
     __syncthreads();
     const __bf16* W = Wt;
     for (int l = 0; l < 4; l++) {
-        floatx16 acc[2][2];
+        floatx16 acc[2][FI_RT];
 #pragma unroll
         for (int f = 0; f < 2; f++)
 #pragma unroll
-            for (int rt = 0; rt < 2; rt++)
+            for (int rt = 0; rt < FI_RT; rt++)
 #pragma unroll
                 for (int j = 0; j < 16; j++) acc[f][rt][j] = 0.f;
-        if (l == 0) fi_layer<16, 2>(W, H, f0, rbase, lane, acc);
-        else fi_layer<FI_W, 2>(W, H, f0, rbase, lane, acc);
+        if (l == 0) fi_layer<16, 2, FI_RT>(W, H, f0, rbase, lane, acc);
+        else fi_layer<FI_W, 2, FI_RT>(W, H, f0, rbase, lane, acc);
         __syncthreads();  // every wave has read this layer's input
-        fi_silu_store(acc, params + boff[l], H, f0, rbase, lane);
+        fi_silu_store<FI_RT>(acc, params + boff[l], H, f0, rbase, lane);
         __syncthreads();
         W += (long)FI_W * (l == 0 ? 16 : FI_W);
     }
-    // output layer: features 0..2 of the first 32-feature tile
-    if (f0 == 0) {
-        floatx16 acc[1][2];
+    // output layer: features 0..2 of the first 32-feature tile, one 32-row
+    // tile per wave at a time
+    const float* bias = params + boff[4];
+    for (int t = w; t < FI_ROWS / 32; t += FI_THREADS / 64) {
+        floatx16 acc[1][1];
 #pragma unroll
-        for (int rt = 0; rt < 2; rt++)
+        for (int j = 0; j < 16; j++) acc[0][0][j] = 0.f;
+        fi_layer<FI_W, 1, 1>(W, H, 0, t * 32, lane, acc);
+        const long gr = r0 + t * 32 + lane;
+        if (lane < 32 && gr < n)
 #pragma unroll
-            for (int j = 0; j < 16; j++) acc[0][rt][j] = 0.f;
-        fi_layer<FI_W, 1>(W, H, 0, rbase, lane, acc);
-        if (lane < 32) {
-            const float* bias = params + boff[4];
-#pragma unroll
-            for (int rt = 0; rt < 2; rt++) {
-                long gr = r0 + rbase + rt * 32 + lane;
-                if (gr < n)
-#pragma unroll
-                    for (int i = 0; i < 3; i++) Y[gr * 3 + i] = acc[0][rt][i] + bias[i];
-            }
-        }
+            for (int i = 0; i < 3; i++) Y[gr * 3 + i] = acc[0][0][i] + bias[i];
     }
 }
 // Wt (BF16, transposed, last layer padded to 32 rows) from the FP32 parameters
