@@ -17,6 +17,7 @@
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 #include "fipc_ctx.h"
 #include "fipc_rows.h"
@@ -238,6 +239,13 @@ __global__ void k_grid_scatter(const int* __restrict__ nep, int cap, const unsig
     }
 }
 
+// A visitor with a prep(q) member loads what it needs of the query once per
+// (query, cell) pair and takes it with each item: visit.item(st, q, item)
+template <class V, class = void>
+struct has_prep : std::false_type {};
+template <class V>
+struct has_prep<V, std::void_t<decltype(&V::prep)>> : std::true_type {};
+
 // Generic query, one thread per (query, cell) pair, grid-stride: visit(q,
 // item) is called once per (query, item) pair whose cell ranges overlap,
 // in the first cell common to both.  Does nothing when the grid build
@@ -275,13 +283,25 @@ __global__ void __launch_bounds__(TPB) k_grid_query(int nq, int np, const int* _
         unsigned long long key = cell_key(a, b, c);
         unsigned bk = bucket_of(a, b, c, M);
         int e0 = bk ? bstart[bk - 1] : 0, e1 = bstart[bk];
-        for (int e = e0; e < e1; e++) {
-            if (cells[e] != key) continue;
-            unsigned it = (unsigned)items[e];
-            unsigned fl = it >> 28;
-            // first common cell: along each axis the cell is the lowest cell of the query or of the item
-            if (!((a == r.lo[0] || (fl & 1u)) && (b == r.lo[1] || (fl & 2u)) && (c == r.lo[2] || (fl & 4u)))) continue;
-            visit(q, (int)(it & 0x0fffffffu));
+        // first common cell: along each axis the cell is the lowest cell of the query or of the item
+        auto first_common = [&](unsigned fl) {
+            return (a == r.lo[0] || (fl & 1u)) && (b == r.lo[1] || (fl & 2u)) && (c == r.lo[2] || (fl & 4u));
+        };
+        if constexpr (has_prep<Visit>::value) {
+            const auto st = visit.prep(q);
+            for (int e = e0; e < e1; e++) {
+                if (cells[e] != key) continue;
+                unsigned it = (unsigned)items[e];
+                if (!first_common(it >> 28)) continue;
+                visit.item(st, q, (int)(it & 0x0fffffffu));
+            }
+        } else {
+            for (int e = e0; e < e1; e++) {
+                if (cells[e] != key) continue;
+                unsigned it = (unsigned)items[e];
+                if (!first_common(it >> 28)) continue;
+                visit(q, (int)(it & 0x0fffffffu));
+            }
         }
     }
 }
@@ -713,18 +733,33 @@ struct CandCCDPT {
     const int *bp, *bt;
     double dHat, eta;
     PairSink out;
-    __device__ __forceinline__ void operator()(int tri, int pi) const
+    // the triangle's vertices and swept box, once per (triangle, cell)
+    struct Q {
+        int t0, t1, t2;
+        v3 mnt, mxt;
+    };
+    __device__ __forceinline__ Q prep(int tri) const
     {
-        int q = bp[pi], t0 = bt[3 * tri], t1 = bt[3 * tri + 1], t2 = bt[3 * tri + 2];
+        Q s;
+        s.t0 = bt[3 * tri]; s.t1 = bt[3 * tri + 1]; s.t2 = bt[3 * tri + 2];
+        v3 T0 = ldv(x, s.t0), T1 = ldv(x, s.t1), T2 = ldv(x, s.t2);
+        v3 d0 = ldv(p, s.t0), d1 = ldv(p, s.t1), d2 = ldv(p, s.t2);
+        s.mxt = vmax(vmax(T0, T0 + d0), vmax(vmax(T1, T1 + d1), vmax(T2, T2 + d2)));
+        s.mnt = vmin(vmin(T0, T0 + d0), vmin(vmin(T1, T1 + d1), vmin(T2, T2 + d2)));
+        return s;
+    }
+    __device__ __forceinline__ void item(const Q& s, int tri, int pi) const
+    {
+        const int q = bp[pi], t0 = s.t0, t1 = s.t1, t2 = s.t2;
         if (q == t0 || q == t1 || q == t2) return;
-        v3 P = ldv(x, q), T0 = ldv(x, t0), T1 = ldv(x, t1), T2 = ldv(x, t2);
-        v3 dP = ldv(p, q), d0 = ldv(p, t0), d1 = ldv(p, t1), d2 = ldv(p, t2);
+        v3 P = ldv(x, q), dP = ldv(p, q);
         // moving_point_triangle_ccd_broadphase (ipc.py:339-345)
         v3 mxp = vmax(P, P + dP), mnp = vmin(P, P + dP);
-        v3 mxt = vmax(vmax(T0, T0 + d0), vmax(vmax(T1, T1 + d1), vmax(T2, T2 + d2)));
-        v3 mnt = vmin(vmin(T0, T0 + d0), vmin(vmin(T1, T1 + d1), vmin(T2, T2 + d2)));
+        const v3 mxt = s.mxt, mnt = s.mnt;
         if (!(mnp.x < mxt.x + dHat && mnp.y < mxt.y + dHat && mnp.z < mxt.z + dHat)) return;
         if (!(mnt.x - dHat < mxp.x && mnt.y - dHat < mxp.y && mnt.z - dHat < mxp.z)) return;
+        v3 T0 = ldv(x, t0), T1 = ldv(x, t1), T2 = ldv(x, t2);
+        v3 d0 = ldv(p, t0), d1 = ldv(p, t1), d2 = ldv(p, t2);
         // the lower bound of pt_ccd, same expressions
         double maxDisp = sqrt(n2(dP)) + sqrt(fmax(fmax(n2(d0), n2(d1)), n2(d2)));
         if (maxDisp == 0) return;
@@ -739,18 +774,32 @@ struct CandCCDEE {
     const int* be;
     double dHat, eta;
     PairSink out;
-    __device__ __forceinline__ void operator()(int i, int j) const
+    // the query edge's vertices and swept box, once per (edge, cell)
+    struct Q {
+        int a0, a1;
+        v3 mna, mxa;
+    };
+    __device__ __forceinline__ Q prep(int i) const
+    {
+        Q s;
+        s.a0 = be[2 * i]; s.a1 = be[2 * i + 1];
+        v3 A0 = ldv(x, s.a0), A1 = ldv(x, s.a1), dA0 = ldv(p, s.a0), dA1 = ldv(p, s.a1);
+        s.mxa = vmax(vmax(A0, A0 + dA0), vmax(A1, A1 + dA1));
+        s.mna = vmin(vmin(A0, A0 + dA0), vmin(A1, A1 + dA1));
+        return s;
+    }
+    __device__ __forceinline__ void item(const Q& s, int i, int j) const
     {
         if (!(i < j)) return;
-        int a0 = be[2 * i], a1 = be[2 * i + 1], b0 = be[2 * j], b1 = be[2 * j + 1];
+        const int a0 = s.a0, a1 = s.a1, b0 = be[2 * j], b1 = be[2 * j + 1];
         if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
-        v3 A0 = ldv(x, a0), A1 = ldv(x, a1), B0 = ldv(x, b0), B1 = ldv(x, b1);
-        v3 dA0 = ldv(p, a0), dA1 = ldv(p, a1), dB0 = ldv(p, b0), dB1 = ldv(p, b1);
+        v3 B0 = ldv(x, b0), B1 = ldv(x, b1), dB0 = ldv(p, b0), dB1 = ldv(p, b1);
         // moving_edge_edge_ccd_broadphase (ipc.py:357-363)
-        v3 mxa = vmax(vmax(A0, A0 + dA0), vmax(A1, A1 + dA1)), mna = vmin(vmin(A0, A0 + dA0), vmin(A1, A1 + dA1));
+        const v3 mxa = s.mxa, mna = s.mna;
         v3 mxb = vmax(vmax(B0, B0 + dB0), vmax(B1, B1 + dB1)), mnb = vmin(vmin(B0, B0 + dB0), vmin(B1, B1 + dB1));
         if (!(mna.x < mxb.x + dHat && mna.y < mxb.y + dHat && mna.z < mxb.z + dHat)) return;
         if (!(mnb.x - dHat < mxa.x && mnb.y - dHat < mxa.y && mnb.z - dHat < mxa.z)) return;
+        v3 A0 = ldv(x, a0), A1 = ldv(x, a1), dA0 = ldv(p, a0), dA1 = ldv(p, a1);
         // the lower bound of ee_ccd, same expressions
         double maxDisp = sqrt(fmax(n2(dA0), n2(dA1))) + sqrt(fmax(n2(dB0), n2(dB1)));
         if (maxDisp == 0) return;
@@ -912,11 +961,23 @@ struct VisitCollide {
     const double* x;
     const int *be, *bt;
     int* hit;
-    __device__ __forceinline__ void operator()(int tri, int ei) const
+    // the triangle, once per (triangle, cell)
+    struct Q {
+        int t0, t1, t2;
+        v3 T0, T1, T2;
+    };
+    __device__ __forceinline__ Q prep(int tri) const
     {
-        int a0 = be[2 * ei], a1 = be[2 * ei + 1], t0 = bt[3 * tri], t1 = bt[3 * tri + 1], t2 = bt[3 * tri + 2];
+        Q s;
+        s.t0 = bt[3 * tri]; s.t1 = bt[3 * tri + 1]; s.t2 = bt[3 * tri + 2];
+        s.T0 = ldv(x, s.t0); s.T1 = ldv(x, s.t1); s.T2 = ldv(x, s.t2);
+        return s;
+    }
+    __device__ __forceinline__ void item(const Q& s, int tri, int ei) const
+    {
+        const int a0 = be[2 * ei], a1 = be[2 * ei + 1], t0 = s.t0, t1 = s.t1, t2 = s.t2;
         if (a0 == t0 || a0 == t1 || a0 == t2 || a1 == t0 || a1 == t1 || a1 == t2) return;
-        if (seg_tri(ldv(x, a0), ldv(x, a1), ldv(x, t0), ldv(x, t1), ldv(x, t2))) atomicOr(hit, 1);
+        if (seg_tri(ldv(x, a0), ldv(x, a1), s.T0, s.T1, s.T2)) atomicOr(hit, 1);
     }
 };
 
